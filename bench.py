@@ -1,0 +1,166 @@
+"""Benchmark of the Phase-1 trial-division hot path on one MI355X.
+
+Workload (the full-scale trial-division configuration): eight seeded (a,b,c)
+triples (tests/golden/configs.json, workload 2), k and j in 1..4096
+(16 777 216 pairs per triple), all 5 761 455 primes below 10^8.  One step
+is one pass over the eight triples: each bitmap is cleared and sieved by
+every prime.  Primes and bitmaps are resident in HBM before the timed region.
+
+Prints one JSON line:  value = pair*primes/s over the whole step
+(8 * 4096^2 * 5 761 455 / seconds per step).  "roofline" rates the dominant
+kernel (the per-prime lane kernel) from its own HIP-event durations on the
+stream it runs on; "cpu_baseline" times the CPU restatement of the
+reference's trial_div_cpu.c on one core, on a bounded sample of the same
+workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import importlib.util
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+K = J = 4096
+DJ = 4
+# Algorithmic INT32 VALU ops of the lane kernel per (prime, k): (1 + D_j)
+# residues each advanced by a 3-op modular subtraction and tested by 1 compare.
+OPS_PER_PRIME_K = 4 * (1 + DJ)
+INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12      # 39.32 Tops/s (256 CUs, wave64, 2.4 GHz)
+SMALL_LIMIT = 1 << 16                               # primes below go to the LDS row kernel
+
+
+def _load(name, path):
+    spec = importlib.util.spec_from_file_location(name, path)
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def cpu_baseline(primes: np.ndarray, triple, budget_s: float) -> dict:
+    """Single-thread CPU restatement of trial_div_cpu.c on a prime sample."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from _support import oracle_sieve
+    a, b, c = triple
+    big = primes[primes >= SMALL_LIMIT]
+    # calibrate on a small slice, then size the sample for ~budget_s
+    probe = big[:: max(1, len(big) // 200)][:200]
+    t0 = time.perf_counter()
+    oracle_sieve(probe, a, b, c, K, J, threads=1)
+    per_prime = (time.perf_counter() - t0) / len(probe)
+    n = int(max(50, min(len(big), budget_s / max(per_prime, 1e-9))))
+    stride = max(1, len(big) // n)
+    sample = big[::stride][:n]
+    t0 = time.perf_counter()
+    oracle_sieve(sample, a, b, c, K, J, threads=1)
+    dt = time.perf_counter() - t0
+    return {"value": K * J * len(sample) / dt, "unit": "pair*primes/s", "cores": 1,
+            "kind": "port",
+            "sample": (f"triple (a,b,c)=({a},{b},{c}), K=J={K}, {len(sample)} primes "
+                       f"(every {stride}th prime >= 2^16 below 10^8), {dt:.1f} s on 1 core; "
+                       f"oracle/td_oracle.c restating trial_div_cpu.c")}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not "
+              f"supported (the triples shard across GPUs with no exchange)", file=sys.stderr)
+        return 2
+
+    pf = _load("pf_amd", ROOT / "prime-finder_amd" / "pf_amd.py")
+    L = pf.lib()
+    assert L.cuda_get_device_count() >= 1, "no HIP device"
+    cfg = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())["2"]
+    triples = cfg["triples"]
+    primes = pf.primes_below(cfg["prime_limit"])
+    n = len(primes)
+    n_large = int((primes >= SMALL_LIMIT).sum())
+    nw = (K * J + 31) // 32
+
+    d_primes = pf.DeviceBuffer.from_array(primes)
+    d_bits = [pf.DeviceBuffer(nw * 4) for _ in triples]
+    evs = [[(L.pf_event_create(), L.pf_event_create()) for _ in triples]
+           for _ in range(args.steps)]
+
+    def step(events=None):
+        for t, (a, b, c) in enumerate(triples):
+            if L.pf_memset(d_bits[t].ptr, 0, nw * 4) != 0:
+                raise RuntimeError("pf_memset failed")
+            e0, e1 = events[t] if events else (None, None)
+            rc = L.trial_div_launch_kj_ev(d_primes.ptr, n, a, b, c, K, J, d_bits[t].ptr,
+                                          None, e0, e1)
+            if rc != 0:
+                raise RuntimeError("trial_div_launch_kj_ev failed")
+
+    for _ in range(args.warmup):
+        step()
+    assert L.pf_sync() == 0
+    assert L.trial_div_check_error() == 0
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        step(evs[s])
+    assert L.pf_sync() == 0
+    wall = time.perf_counter() - t0
+    assert L.trial_div_check_error() == 0
+
+    lane_ms = [L.pf_event_elapsed_ms(e0, e1) for row in evs for (e0, e1) in row]
+    for row in evs:
+        for e0, e1 in row:
+            L.pf_event_destroy(e0)
+            L.pf_event_destroy(e1)
+    avg_lane_s = float(np.mean(lane_ms)) / 1e3
+    ms_per_step = wall / args.steps * 1e3
+    work = len(triples) * K * J * n
+    value = work / (wall / args.steps)
+    achieved = n_large * K * OPS_PER_PRIME_K / avg_lane_s / 1e12
+
+    bits0 = d_bits[0].download()
+    survivors0 = K * J - int(np.unpackbits(bits0.view(np.uint8)).sum())
+
+    out = {
+        "metric": "trial-div pair*primes/s",
+        "value": value,
+        "unit": "pair*primes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded triples, tests/golden/configs.json workload 2)",
+        "config": {"workload": "trial division: 8 triples, K=J=4096, primes < 1e8",
+                   "triples": len(triples), "K": K, "J": J, "primes": n,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": INT32_VALU_PEAK_TOPS,
+                     "unit": "Tops/s", "frac": achieved / INT32_VALU_PEAK_TOPS,
+                     "traffic": None,
+                     "kernel": "td_large_kernel<4>",
+                     "kernel_avg_ms": avg_lane_s * 1e3,
+                     "ops_per_launch": n_large * K * OPS_PER_PRIME_K},
+        "survivors_first_triple": survivors0,
+        "device": pf.device_info(),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(primes, triples[0], args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,109 @@
+/*
+ * trial_div.h -- Phase-1 trial division on an MI355X (gfx950), C ABI.
+ *
+ * Drop-in for the reference's trial_div.h: the first four declarations keep
+ * the reference's names, argument meaning, return codes and the
+ * OR-into-caller-bits contract, so worker.c (worker.c:37-42), main.c
+ * (main.c:92-94) and tests/test_gpu.c call them unchanged.  The cuda_* names
+ * are kept for ABI compatibility; they are implemented over HIP.
+ *
+ * Bitmap contract: bit (k-1)*J + (j-1), LSB-first in uint32 words,
+ * 1 = (k,j) eliminated because some prime p of the list divides
+ *     N = 10^a - k 10^b - j 10^c - 1, or divides
+ *     rev(N) = 10^a - rev(k) 10^(a-b-lk) - rev(j) 10^(a-c-lj) - 1
+ *     (rev(x) = decimal reversal of x, lx = its digit count; a rev term with
+ *     a negative exponent eliminates nothing).  Identical to the reference's
+ *     CPU sieve trial_div_cpu.c:32-111 on every input.
+ */
+#pragma once
+#include "prime_list.h"
+#include <stddef.h>
+#include <stdint.h>
+
+/* The reference's fixed grid (sieve_table.h:15-17): K = J = 100. The
+ * definitions are token-identical to the reference's so that including both
+ * headers in one translation unit is a benign redefinition. */
+#define BITS_PER_WORD   32
+#define KJ_BITS         10000
+#define KJ_WORDS        ((KJ_BITS + BITS_PER_WORD - 1) / BITS_PER_WORD)
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    double upload;    /* host->device copies (bits; primes when not cached), s */
+    double kernel;    /* all trial-division kernels, s                        */
+    double download;  /* device->host copy of the bits, s                     */
+} gpu_timing_t;       /* reference trial_div.h:10-14 */
+
+/* Replaces reference trial_div.h:26-29 / trial_div.cu:155-224.
+ * K = J = 100; sieve_bits holds KJ_WORDS words, in/out (OR-accumulate).
+ * Synchronous.  Returns 0, or -1 after printing
+ * "trial_div_gpu: HIP error ..." to stderr (bits then unchanged).
+ * The device copy of pl->primes is cached across calls and refreshed when
+ * the pointer, the count or the contents (64-bit checksum) change.
+ * Every prime must satisfy 2 <= p < 2^31. */
+int trial_div_gpu(const prime_list_t *pl,
+                  long long a, long long b, long long c,
+                  uint32_t *sieve_bits,
+                  gpu_timing_t *timing);
+
+/* Replaces reference trial_div.h:31 (trial_div.cu:137-140): number of HIP
+ * devices, 0 on error. */
+int cuda_get_device_count(void);
+
+/* Replaces reference trial_div.h:32 (trial_div.cu:142-145): 0 / -1. */
+int cuda_set_device(int index);
+
+/* Extension of trial_div_gpu to a K x J grid (1 <= K, J <= 9999), for the
+ * 4096 x 4096 grids the reference's fixed 100 x 100 cannot express
+ * (sieve_table.h:15-17).  bits holds n_words >= ceil(K*J/32) words, in/out.
+ * Same semantics, caching and error convention as trial_div_gpu. */
+int trial_div_gpu_kj(const prime_list_t *pl,
+                     long long a, long long b, long long c,
+                     uint32_t K, uint32_t J,
+                     uint32_t *bits, size_t n_words,
+                     gpu_timing_t *timing);
+
+/* Device-resident form, asynchronous on `stream` (a hipStream_t, NULL =
+ * the null stream): d_primes (n_primes words) and d_bits (ceil(K*J/32)
+ * words, OR-accumulated) are device pointers.  This is the generalised
+ * trial_div_launch (reference trial_div.cu:127-133) that the benchmark and
+ * batch callers use; it returns -1 only for invalid arguments or launch
+ * failure.  Input-data errors (a prime outside [2, 2^31)) are latched on the
+ * device and reported by trial_div_check_error().  Calls on one device share
+ * scratch memory: issue them from one host thread, on one stream. */
+int trial_div_launch_kj(const uint32_t *d_primes, uint32_t n_primes,
+                        long long a, long long b, long long c,
+                        uint32_t K, uint32_t J,
+                        uint32_t *d_bits, void *stream);
+
+/* trial_div_launch_kj that also records two hipEvent_t on `stream` around
+ * its dominant kernel (the per-prime lane kernel for primes >= 2^16), so a
+ * benchmark can time that kernel on the stream it runs on.  Either event may
+ * be NULL.  No counterpart in the reference (its bench times the whole
+ * launch, tests/bench_gpu.cu:53-62). */
+int trial_div_launch_kj_ev(const uint32_t *d_primes, uint32_t n_primes,
+                           long long a, long long b, long long c,
+                           uint32_t K, uint32_t J,
+                           uint32_t *d_bits, void *stream,
+                           void *ev_lane_start, void *ev_lane_stop);
+
+/* Synchronises the current device, then returns and clears the error latched
+ * by earlier launches: 0 = none, 1 = a prime outside [2, 2^31),
+ * 2 = more than 65536 list entries below 2^16 (a duplicated list). */
+int trial_div_check_error(void);
+
+#ifdef __cplusplus
+} /* extern "C" */
+
+/* The reference's configurable launcher keeps its C++ linkage
+ * (trial_div.cu:127-133, declared by its caller tests/bench_gpu.cu:25-27):
+ * K = J = 100 on device buffers, asynchronous on the null stream.
+ * block_size is accepted for signature compatibility; the launch shapes are
+ * fixed by this implementation. */
+void trial_div_launch(const uint32_t *d_primes, uint32_t n_primes,
+                      uint64_t a, uint64_t b, uint64_t c,
+                      uint32_t *d_bits, int block_size);
+#endif

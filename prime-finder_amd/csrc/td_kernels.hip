@@ -1,0 +1,638 @@
+// td_kernels.hip -- Phase-1 trial division for gfx950.
+//
+// Problem (reference trial_div.cu:46-123, trial_div_cpu.c:32-111): for every
+// prime p of a list and every (k, j) of a K x J grid, eliminate (k, j) when
+//   p | N      = 10^a - k 10^b - j 10^c - 1,                  or
+//   p | rev(N) = 10^a - rev(k) 10^(a-b-lk) - rev(j) 10^(a-c-lj) - 1,
+// with the reference CPU sieve's gating (trial_div_cpu.c:41,57-63,68,73,82,97):
+// no normal solve when 10^c == 0 (mod p), no rev solve for a k (or j digit
+// group) whose exponent is negative or whose power of ten vanishes mod p.
+//
+// Both conditions are linear in one unknown once the other is fixed, so per
+// prime and row k there is one residue u per condition whose small values
+// are the hits (u = j - 1 for N, u = rev(j) - 1 for each j digit group).
+// Along k these residues move by fixed steps, so after a per-prime setup the
+// k loop needs only modular subtractions and compares: no multiplications.
+//
+// Work split:
+//   td_split_kernel        wave-ballot compaction of primes < 2^16 into a
+//                          scratch list (one atomic per wave), and input
+//                          validation;
+//   td_small_setup_kernel  per small prime: the residues and digit-group
+//                          constants, with the reference's exact arithmetic
+//                          (Fermat inverses, the powmod conventions);
+//   td_rows_kernel         small primes hit many j per row, so a workgroup
+//                          owns a band of TD_ROWS rows in LDS, lanes spread
+//                          over (row, prime), marks are LDS atomics and the
+//                          band is OR-ed into HBM once;
+//   td_large_kernel        one lane per prime >= 2^16: Montgomery setup, then
+//                          a k loop of 3-op modular steps with a rare
+//                          divergent hit path (global atomic OR).
+#include "pf_hip.h"
+#include "../../include/trial_div.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+namespace {
+
+constexpr uint32_t TD_SMALL_LIMIT = 1u << 16;   // row kernel below, lane kernel above
+constexpr uint32_t TD_SMALL_CAP   = 65536;      // list entries below TD_SMALL_LIMIT
+constexpr uint32_t TD_TBL         = 32;         // uint32 per small-prime table entry
+constexpr uint32_t TD_ROWS        = 16;         // rows per row-kernel workgroup
+constexpr uint32_t TD_BLOCK       = 256;
+constexpr uint32_t TD_PMAX        = 0x80000000u;
+constexpr int      TD_SETUP_GRID  = 32;
+
+// Small-prime table entry (TD_TBL words):
+enum {
+    TT_P = 0, TT_MAGIC = 1, TT_FLAGS = 2, TT_UN0 = 3, TT_B = 4,
+    TT_CM = 5,   // 4 words: Cm[lj-1]
+    TT_E = 9,    // 16 words: E[(lk-1)*4 + (lj-1)]
+};
+
+// ctrl words
+enum { CT_SMALL_COUNT = 0, CT_ERROR = 1, CT_WORDS = 4 };
+
+__device__ __forceinline__ uint32_t td_pow10(int e)
+{
+    // e in [0, 5]; a select chain keeps it in registers.
+    uint32_t v = 1u;
+    v = e >= 1 ? 10u : v;
+    v = e >= 2 ? 100u : v;
+    v = e >= 3 ? 1000u : v;
+    v = e >= 4 ? 10000u : v;
+    v = e >= 5 ? 100000u : v;
+    return v;
+}
+
+__device__ __forceinline__ int td_ndigits(uint32_t x)
+{
+    return 1 + (x >= 10u) + (x >= 100u) + (x >= 1000u) + (x >= 10000u);
+}
+
+// Reverse the lowest `len` decimal digits of x, leading zeros included.
+__device__ __forceinline__ uint32_t td_reverse(uint32_t x, int len)
+{
+    uint32_t out = 0;
+    for (int i = 0; i < len; i++) {
+        uint32_t q = x / 10u;
+        out = out * 10u + (x - q * 10u);
+        x = q;
+    }
+    return out;
+}
+
+// x mod p for x < 2^32 with magic = floor(2^32 / p).
+__device__ __forceinline__ uint32_t td_mod(uint32_t x, uint32_t p, uint32_t magic)
+{
+    uint32_t q = __umulhi(x, magic);
+    uint32_t r = x - q * p;
+    return r >= p ? r - p : r;
+}
+
+// The reference's square-and-multiply (trial_div_cpu.c:22-30) for p < 2^16,
+// where every product fits in 32 bits.
+__device__ uint32_t td_powmod16(uint32_t base, uint64_t e, uint32_t p)
+{
+    uint32_t acc = 1u, sq = base % p;
+    while (e) {
+        if (e & 1u) acc = acc * sq % p;
+        sq = sq * sq % p;
+        e >>= 1;
+    }
+    return acc;
+}
+
+__global__ void td_reset_kernel(uint32_t *ctrl)
+{
+    if (threadIdx.x == 0) ctrl[CT_SMALL_COUNT] = 0u;
+}
+
+__global__ __launch_bounds__(TD_BLOCK)
+void td_split_kernel(const uint32_t *__restrict__ primes, uint32_t n,
+                     uint32_t *__restrict__ small, uint32_t *__restrict__ ctrl)
+{
+    const uint32_t i = blockIdx.x * TD_BLOCK + threadIdx.x;
+    const uint32_t p = i < n ? primes[i] : TD_SMALL_LIMIT;
+    const bool bad = i < n && (p < 2u || p >= TD_PMAX);
+    const bool is_small = !bad && p < TD_SMALL_LIMIT;
+    if (bad) atomicOr(&ctrl[CT_ERROR], 1u);
+    const uint64_t mask = __ballot(is_small);
+    if (mask == 0) return;
+    const int lane = __lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(&ctrl[CT_SMALL_COUNT], (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    if (is_small) {
+        const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        if (pos < TD_SMALL_CAP) small[pos] = p;
+        else atomicOr(&ctrl[CT_ERROR], 2u);
+    }
+}
+
+__global__ __launch_bounds__(TD_BLOCK)
+void td_small_setup_kernel(const uint32_t *__restrict__ small,
+                           const uint32_t *__restrict__ ctrl,
+                           long long a, long long b, long long c,
+                           uint32_t *__restrict__ tbl)
+{
+    const uint32_t count = min(ctrl[CT_SMALL_COUNT], TD_SMALL_CAP);
+    for (uint32_t i = blockIdx.x * TD_BLOCK + threadIdx.x; i < count;
+         i += gridDim.x * TD_BLOCK) {
+        const uint32_t p = small[i];
+        uint32_t *e = tbl + (size_t)i * TD_TBL;
+        const uint32_t ra = td_powmod16(10u, (uint64_t)a, p);
+        const uint32_t rb = td_powmod16(10u, (uint64_t)b, p);
+        const uint32_t rc = td_powmod16(10u, (uint64_t)c, p);
+        const uint32_t ra1 = (ra + p - 1u) % p;
+        uint32_t flags = 0;
+        uint32_t un0 = 0, bstep = 0;
+        if (rc != 0u) {
+            const uint32_t ic = td_powmod16(rc, p - 2u, p);
+            un0 = (ra1 * ic % p + p - 1u) % p;   // ((ra-1)/rc - 1) mod p
+            bstep = rb * ic % p;                 // rb/rc mod p
+            flags |= 1u;
+        }
+        uint32_t kres[4], jinv[4];
+        for (int l = 1; l <= 4; l++) {
+            const long long eb = a - b - l, ec = a - c - l;
+            kres[l - 1] = eb >= 0 ? td_powmod16(10u, (uint64_t)eb, p) : 0u;
+            const uint32_t jr = ec >= 0 ? td_powmod16(10u, (uint64_t)ec, p) : 0u;
+            jinv[l - 1] = jr ? td_powmod16(jr, p - 2u, p) : 0u;
+        }
+        e[TT_P] = p;
+        e[TT_MAGIC] = (uint32_t)(0x100000000ull / p);
+        e[TT_UN0] = un0;
+        e[TT_B] = bstep;
+        for (int lj = 0; lj < 4; lj++)
+            e[TT_CM + lj] = (ra1 * jinv[lj] % p + p - 1u) % p;
+        for (int lk = 0; lk < 4; lk++)
+            for (int lj = 0; lj < 4; lj++) {
+                e[TT_E + lk * 4 + lj] = kres[lk] * jinv[lj] % p;
+                if (kres[lk] != 0u && jinv[lj] != 0u) flags |= 1u << (1 + lk * 4 + lj);
+            }
+        e[TT_FLAGS] = flags;
+    }
+}
+
+__device__ __forceinline__ void td_lds_mark(uint32_t *lds, uint32_t bit)
+{
+    atomicOr(&lds[bit >> 5], 1u << (bit & 31u));
+}
+
+template <int DJ>
+__global__ __launch_bounds__(TD_BLOCK)
+void td_rows_kernel(const uint32_t *__restrict__ tbl, const uint32_t *__restrict__ ctrl,
+                    uint32_t K, uint32_t J, uint32_t *__restrict__ bits)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t k0 = blockIdx.x * TD_ROWS + 1u;
+    const uint32_t k_last = min(k0 + TD_ROWS - 1u, K);
+    const uint32_t w0 = ((k0 - 1u) * J) >> 5;
+    const uint32_t nw = ((k_last * J - 1u) >> 5) - w0 + 1u;
+    for (uint32_t w = threadIdx.x; w < nw; w += TD_BLOCK) lds[w] = 0u;
+    __syncthreads();
+
+    const uint32_t count = min(ctrl[CT_SMALL_COUNT], TD_SMALL_CAP);
+    const uint32_t row = threadIdx.x % TD_ROWS;
+    const uint32_t slot = threadIdx.x / TD_ROWS;
+    constexpr uint32_t nslots = TD_BLOCK / TD_ROWS;
+    const uint32_t k = k0 + row;
+    if (k <= K) {
+        const int lk = td_ndigits(k);
+        const uint32_t rk = td_reverse(k, lk);
+        // LDS-relative bit of (k, j) is rowbit + j.
+        const uint32_t rowbit = (k - 1u) * J - (w0 << 5) - 1u;
+        for (uint32_t i = slot; i < count; i += nslots) {
+            const uint32_t *e = tbl + (size_t)i * TD_TBL;
+            const uint32_t p = e[TT_P], magic = e[TT_MAGIC], flags = e[TT_FLAGS];
+            if (flags & 1u) {
+                const uint32_t u = pf_submod(e[TT_UN0], td_mod(k * e[TT_B], p, magic), p);
+                for (uint32_t j = u + 1u; j <= J; j += p) td_lds_mark(lds, rowbit + j);
+            }
+#pragma unroll
+            for (int lj = 1; lj <= DJ; lj++) {
+                if (!((flags >> (1 + (lk - 1) * 4 + (lj - 1))) & 1u)) continue;
+                const uint32_t ev = e[TT_E + (lk - 1) * 4 + (lj - 1)];
+                const uint32_t u = pf_submod(e[TT_CM + lj - 1], td_mod(rk * ev, p, magic), p);
+                const uint32_t lim = td_pow10(lj);
+                for (uint32_t r = u + 1u; r < lim; r += p) {
+                    if (r % 10u == 0u) continue;
+                    const uint32_t j = td_reverse(r, lj);
+                    if (j <= J) td_lds_mark(lds, rowbit + j);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < nw; w += TD_BLOCK) {
+        const uint32_t v = lds[w];
+        if (v) atomicOr(&bits[w0 + w], v);
+    }
+}
+
+__device__ __forceinline__ void td_gmark(uint32_t *bits, uint32_t bit)
+{
+    __hip_atomic_fetch_or(&bits[bit >> 5], 1u << (bit & 31u), __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Rare path of the lane kernel: emit every hit of row k.
+template <int DJ>
+__device__ __forceinline__ void td_hits(uint32_t k, uint32_t un, uint32_t thrn,
+                                        const uint32_t (&v)[DJ], const uint32_t (&thr)[DJ],
+                                        uint32_t p, uint32_t J, uint32_t *bits)
+{
+    const uint32_t rowbit = (k - 1u) * J - 1u;      // bit of (k, j) = rowbit + j
+    if (un < thrn)
+        for (uint32_t j = un + 1u; j <= J; j += p) td_gmark(bits, rowbit + j);
+#pragma unroll
+    for (int l = 0; l < DJ; l++) {
+        if (v[l] < thr[l]) {
+            const uint32_t lim = td_pow10(l + 1);
+            for (uint32_t r = v[l] + 1u; r < lim; r += p) {
+                if (r % 10u == 0u) continue;
+                const uint32_t j = td_reverse(r, l + 1);
+                if (j <= J) td_gmark(bits, rowbit + j);
+            }
+        }
+    }
+}
+
+template <int DJ>
+__global__ __launch_bounds__(TD_BLOCK)
+void td_large_kernel(const uint32_t *__restrict__ primes, uint32_t n,
+                     long long a, long long b, long long c,
+                     uint32_t K, uint32_t J, uint32_t *__restrict__ bits)
+{
+    const uint32_t i = blockIdx.x * TD_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = primes[i];
+    if (p < TD_SMALL_LIMIT || p >= TD_PMAX) return;
+
+    // ---- per-prime setup (Montgomery, R = 2^32) ----
+    const PfMont m = pf_mont_init(p);
+    const uint32_t ten = pf_mont_in(10u, m);
+    // 10^{-1}: 10 x = 1 + t p with t p == 9 (mod 10), t fixed by p mod 10.
+    const uint32_t pm = p % 10u;
+    const uint32_t t = pm == 1u ? 9u : pm == 3u ? 3u : pm == 7u ? 7u : 1u;
+    const uint32_t tenInv = pf_mont_in((uint32_t)((1ull + (uint64_t)t * p) / 10u), m);
+
+    const uint32_t raM = pf_mont_pow(ten, (uint64_t)a, m);
+    const uint32_t rbM = pf_mont_pow(ten, (uint64_t)b, m);
+    const uint32_t icS = pf_mont_out(pf_mont_pow(tenInv, (uint64_t)c, m), m);   // 10^-c
+    const uint32_t ra1M = pf_submod(raM, m.one, p);                              // ra - 1
+    const uint32_t bstep = pf_mont_mul(rbM, icS, m);                             // rb / rc
+    uint32_t un = pf_submod(pf_mont_mul(ra1M, icS, m), 1u, p);                   // u at k = 0
+    const uint32_t thrn = J;
+
+    // j digit groups: 10^{-(a-c-l)} (Montgomery) and Cm = (ra-1) 10^{-(a-c-l)} - 1.
+    uint32_t jinvM[DJ], cm[DJ];
+    bool jok[DJ];
+    {
+        const long long e1 = a - c - 1;
+        uint32_t z = e1 >= 0 ? pf_mont_pow(tenInv, (uint64_t)e1, m) : 0u;
+#pragma unroll
+        for (int l = 0; l < DJ; l++) {
+            jok[l] = (a - c - (l + 1)) >= 0;
+            jinvM[l] = z;
+            cm[l] = pf_submod(pf_mont_mul(ra1M, pf_mont_out(z, m), m), 1u, p);
+            z = pf_mont_mul(z, ten, m);
+        }
+    }
+    // k digit groups: kres = 10^(a-b-lk), advanced by 10^{-1} per group.
+    const long long eb1 = a - b - 1;
+    uint32_t kresM = eb1 >= 0 ? pf_mont_pow(ten, (uint64_t)eb1, m) : 0u;
+
+    const int DK = (K >= 1000u) ? 4 : (K >= 100u) ? 3 : (K >= 10u) ? 2 : 1;
+    for (int lk = 1; lk <= DK; lk++) {
+        const uint32_t klo = td_pow10(lk - 1);
+        const uint32_t khi = min(K, td_pow10(lk) - 1u);
+        const bool kok = (a - b - lk) >= 0;
+        // Steps of rev(k) inside a row of ten (units digit of k -> top digit
+        // of rev(k)) and between rows (q -> q+1 with t9 trailing nines).
+        const uint32_t s0w = td_pow10(lk - 1);
+        uint32_t thr[DJ], s0[DJ], w[DJ], tq0[DJ], tq1[DJ], tq2[DJ];
+#pragma unroll
+        for (int l = 0; l < DJ; l++) {
+            const uint32_t eM = pf_mont_mul(kresM, jinvM[l], m);      // E = kres/jres
+            thr[l] = (kok && jok[l]) ? td_pow10(l + 1) - 1u : 0u;
+            s0[l] = pf_mont_mul(eM, s0w, m);
+            const uint32_t eS = pf_mont_out(eM, m);
+            w[l] = lk == 1 ? cm[l] : pf_submod(cm[l], eS, p);         // rev(10^(lk-1)) = 1
+            // delta_t = 10^(lk-2-t) - 9 * sum_{i<t} 10^(lk-2-i), reduced mod p.
+            int d0 = lk >= 2 ? (int)td_pow10(lk - 2) : 0;
+            int d1 = lk >= 3 ? (int)td_pow10(lk - 3) - 9 * (int)td_pow10(lk - 2) : 0;
+            int d2 = lk >= 4 ? (int)td_pow10(lk - 4) - 9 * ((int)td_pow10(lk - 2) + (int)td_pow10(lk - 3)) : 0;
+            tq0[l] = pf_mont_mul(eM, d0 >= 0 ? (uint32_t)d0 : p - (uint32_t)(-d0), m);
+            tq1[l] = pf_mont_mul(eM, d1 >= 0 ? (uint32_t)d1 : p - (uint32_t)(-d1), m);
+            tq2[l] = pf_mont_mul(eM, d2 >= 0 ? (uint32_t)d2 : p - (uint32_t)(-d2), m);
+        }
+
+        const uint32_t qlo = klo / 10u, qhi = khi / 10u;
+        for (uint32_t q = qlo; q <= qhi; q++) {
+            const uint32_t kb = q * 10u;
+            const uint32_t dlo = kb < klo ? klo - kb : 0u;
+            const uint32_t dhi = min(9u, khi - kb);
+            uint32_t v[DJ];
+#pragma unroll
+            for (int l = 0; l < DJ; l++) v[l] = w[l];
+            if (dlo == 0u && dhi == 9u) {
+#pragma unroll
+                for (uint32_t d = 0; d < 10u; d++) {
+                    un = pf_submod(un, bstep, p);
+                    bool h = un < thrn;
+#pragma unroll
+                    for (int l = 0; l < DJ; l++) h |= v[l] < thr[l];
+                    if (__builtin_expect(h, 0)) td_hits<DJ>(kb + d, un, thrn, v, thr, p, J, bits);
+#pragma unroll
+                    for (int l = 0; l < DJ; l++) v[l] = pf_submod(v[l], s0[l], p);
+                }
+            } else {
+                for (uint32_t d = 0; d < dlo; d++) {
+#pragma unroll
+                    for (int l = 0; l < DJ; l++) v[l] = pf_submod(v[l], s0[l], p);
+                }
+                for (uint32_t d = dlo; d <= dhi; d++) {
+                    un = pf_submod(un, bstep, p);
+                    bool h = un < thrn;
+#pragma unroll
+                    for (int l = 0; l < DJ; l++) h |= v[l] < thr[l];
+                    if (h) td_hits<DJ>(kb + d, un, thrn, v, thr, p, J, bits);
+#pragma unroll
+                    for (int l = 0; l < DJ; l++) v[l] = pf_submod(v[l], s0[l], p);
+                }
+            }
+            if (q < qhi) {
+                uint32_t t9 = 0, qq = q;
+                while (qq % 10u == 9u) { t9++; qq /= 10u; }
+#pragma unroll
+                for (int l = 0; l < DJ; l++)
+                    w[l] = pf_submod(w[l], t9 == 0 ? tq0[l] : t9 == 1 ? tq1[l] : tq2[l], p);
+            }
+        }
+        kresM = pf_mont_mul(kresM, tenInv, m);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+constexpr int TD_MAX_DEV = 64;
+
+struct TdScratch {
+    uint32_t *small = nullptr;   // [TD_SMALL_CAP]
+    uint32_t *tbl = nullptr;     // [TD_SMALL_CAP * TD_TBL]
+    uint32_t *ctrl = nullptr;    // [CT_WORDS]
+};
+
+TdScratch g_scratch[TD_MAX_DEV];
+
+int td_scratch(TdScratch **out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= TD_MAX_DEV) return -1;
+    TdScratch &s = g_scratch[dev];
+    if (!s.ctrl) {
+        if (hipMalloc(&s.small, TD_SMALL_CAP * sizeof(uint32_t)) != hipSuccess) return -1;
+        if (hipMalloc(&s.tbl, (size_t)TD_SMALL_CAP * TD_TBL * sizeof(uint32_t)) != hipSuccess) return -1;
+        if (hipMalloc(&s.ctrl, CT_WORDS * sizeof(uint32_t)) != hipSuccess) return -1;
+        if (hipMemset(s.ctrl, 0, CT_WORDS * sizeof(uint32_t)) != hipSuccess) return -1;
+    }
+    *out = &s;
+    return 0;
+}
+
+template <int DJ>
+void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long b,
+                    long long c, uint32_t K, uint32_t J, uint32_t *d_bits,
+                    const TdScratch &s, hipStream_t st, hipEvent_t ev0, hipEvent_t ev1)
+{
+    const uint32_t row_groups = (K + TD_ROWS - 1u) / TD_ROWS;
+    const size_t lds = ((size_t)TD_ROWS * J / 32u + 2u) * sizeof(uint32_t);
+    td_rows_kernel<DJ><<<row_groups, TD_BLOCK, lds, st>>>(s.tbl, s.ctrl, K, J, d_bits);
+    if (ev0) (void)hipEventRecord(ev0, st);
+    if (n)
+        td_large_kernel<DJ><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
+            d_primes, n, a, b, c, K, J, d_bits);
+    if (ev1) (void)hipEventRecord(ev1, st);
+}
+
+int td_launch(const uint32_t *d_primes, uint32_t n, long long a, long long b, long long c,
+              uint32_t K, uint32_t J, uint32_t *d_bits, hipStream_t st,
+              hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr)
+{
+    if (K < 1u || J < 1u || K > 9999u || J > 9999u) return -1;
+    if (a < 0 || b < 0 || c < 0) return -1;
+    if (!d_bits || (n && !d_primes)) return -1;
+    TdScratch *s = nullptr;
+    if (td_scratch(&s) != 0) return -1;
+    td_reset_kernel<<<1, 64, 0, st>>>(s->ctrl);
+    if (n)
+        td_split_kernel<<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
+            d_primes, n, s->small, s->ctrl);
+    td_small_setup_kernel<<<TD_SETUP_GRID, TD_BLOCK, 0, st>>>(s->small, s->ctrl, a, b, c, s->tbl);
+    const int dj = J >= 1000u ? 4 : J >= 100u ? 3 : J >= 10u ? 2 : 1;
+    switch (dj) {
+    case 1: td_launch_rest<1>(d_primes, n, a, b, c, K, J, d_bits, *s, st, ev0, ev1); break;
+    case 2: td_launch_rest<2>(d_primes, n, a, b, c, K, J, d_bits, *s, st, ev0, ev1); break;
+    case 3: td_launch_rest<3>(d_primes, n, a, b, c, K, J, d_bits, *s, st, ev0, ev1); break;
+    default: td_launch_rest<4>(d_primes, n, a, b, c, K, J, d_bits, *s, st, ev0, ev1); break;
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Device copy of a host prime list, reused while (pointer, count, checksum)
+// are unchanged.
+struct TdPrimeCache {
+    const uint32_t *host = nullptr;
+    uint32_t count = 0;
+    uint64_t sum = 0;
+    uint32_t *dev = nullptr;
+    size_t cap = 0;
+    uint32_t *bits = nullptr;
+    size_t bits_cap = 0;
+};
+
+TdPrimeCache g_cache[TD_MAX_DEV];
+
+uint64_t td_checksum(const uint32_t *v, uint32_t n)
+{
+    uint64_t h0 = 0x9E3779B97F4A7C15ull, h1 = n;
+    for (uint32_t i = 0; i < n; i++) {
+        h0 = (h0 ^ v[i]) * 0x100000001B3ull;
+        h1 += (uint64_t)v[i] * (2u * i + 1u);
+    }
+    return h0 ^ (h1 << 1);
+}
+
+double td_secs(hipEvent_t x, hipEvent_t y)
+{
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, x, y);
+    return (double)ms * 1e-3;
+}
+
+int td_check_error_code()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= TD_MAX_DEV) return -1;
+    TdScratch &s = g_scratch[dev];
+    if (!s.ctrl) return 0;
+    uint32_t err = 0;
+    if (hipMemcpy(&err, s.ctrl + CT_ERROR, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    if (err) {
+        const uint32_t zero = 0;
+        if (hipMemcpy(s.ctrl + CT_ERROR, &zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess)
+            return -1;
+    }
+    return (int)err;
+}
+
+int td_run_host(const prime_list_t *pl, long long a, long long b, long long c,
+                uint32_t K, uint32_t J, uint32_t *bits, size_t n_words,
+                gpu_timing_t *timing, const char *who)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+    int dev = 0;
+    int rc = -1;
+    const size_t need = ((size_t)K * J + 31u) / 32u;
+    const size_t bb = need * sizeof(uint32_t);
+    TdPrimeCache *pc = nullptr;
+    bool fresh = false;
+    if (timing) *timing = gpu_timing_t{0.0, 0.0, 0.0};
+    if (!pl || !bits || n_words < need || K < 1u || J < 1u || K > 9999u || J > 9999u ||
+        a < 0 || b < 0 || c < 0 || (pl->count && !pl->primes)) {
+        fprintf(stderr, "%s: invalid arguments\n", who);
+        return -1;
+    }
+    PF_TRY(who, hipGetDevice(&dev));
+    if (dev < 0 || dev >= TD_MAX_DEV) goto fail;
+    pc = &g_cache[dev];
+    PF_TRY(who, hipEventCreate(&e0));
+    PF_TRY(who, hipEventCreate(&e1));
+    PF_TRY(who, hipEventCreate(&e2));
+    PF_TRY(who, hipEventCreate(&e3));
+    {
+        const uint64_t sum = td_checksum(pl->primes, pl->count);
+        fresh = !(pc->dev && pc->host == pl->primes && pc->count == pl->count && pc->sum == sum);
+        if (fresh) {
+            pc->host = nullptr;  // invalid until the copy below succeeds
+            if (pc->cap < pl->count) {
+                if (pc->dev) PF_TRY(who, hipFree(pc->dev));
+                pc->dev = nullptr;
+                pc->cap = 0;
+                PF_TRY(who, hipMalloc(&pc->dev, (size_t)pl->count * sizeof(uint32_t) + 4));
+                pc->cap = pl->count;
+            }
+            if (!pc->dev) PF_TRY(who, hipMalloc(&pc->dev, 4));
+        }
+        if (pc->bits_cap < need) {
+            if (pc->bits) PF_TRY(who, hipFree(pc->bits));
+            pc->bits = nullptr;
+            pc->bits_cap = 0;
+            PF_TRY(who, hipMalloc(&pc->bits, bb));
+            pc->bits_cap = need;
+        }
+        PF_TRY(who, hipEventRecord(e0, nullptr));
+        if (fresh && pl->count)
+            PF_TRY(who, hipMemcpy(pc->dev, pl->primes, (size_t)pl->count * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice));
+        PF_TRY(who, hipMemcpy(pc->bits, bits, bb, hipMemcpyHostToDevice));
+        if (fresh) {
+            pc->host = pl->primes;
+            pc->count = pl->count;
+            pc->sum = sum;
+        }
+        PF_TRY(who, hipEventRecord(e1, nullptr));
+        if (td_launch(pc->dev, pl->count, a, b, c, K, J, pc->bits, nullptr) != 0) {
+            fprintf(stderr, "%s: HIP error (kernel launch)\n", who);
+            goto fail;
+        }
+        PF_TRY(who, hipEventRecord(e2, nullptr));
+        PF_TRY(who, hipEventSynchronize(e2));
+        const int err = td_check_error_code();
+        if (err != 0) {
+            fprintf(stderr, "%s: %s\n", who,
+                    err == 1 ? "prime outside [2, 2^31) in list"
+                    : err == 2 ? "more than 65536 list entries below 2^16"
+                               : "HIP error (error flag readback)");
+            goto fail;
+        }
+        PF_TRY(who, hipMemcpy(bits, pc->bits, bb, hipMemcpyDeviceToHost));
+        PF_TRY(who, hipEventRecord(e3, nullptr));
+        PF_TRY(who, hipEventSynchronize(e3));
+        if (timing) {
+            timing->upload = td_secs(e0, e1);
+            timing->kernel = td_secs(e1, e2);
+            timing->download = td_secs(e2, e3);
+        }
+        rc = 0;
+    }
+fail:
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e2) (void)hipEventDestroy(e2);
+    if (e3) (void)hipEventDestroy(e3);
+    return rc;
+}
+
+} // namespace
+
+extern "C" int trial_div_gpu(const prime_list_t *pl, long long a, long long b, long long c,
+                             uint32_t *sieve_bits, gpu_timing_t *timing)
+{
+    return td_run_host(pl, a, b, c, 100u, 100u, sieve_bits, KJ_WORDS, timing, "trial_div_gpu");
+}
+
+extern "C" int trial_div_gpu_kj(const prime_list_t *pl, long long a, long long b, long long c,
+                                uint32_t K, uint32_t J, uint32_t *bits, size_t n_words,
+                                gpu_timing_t *timing)
+{
+    return td_run_host(pl, a, b, c, K, J, bits, n_words, timing, "trial_div_gpu_kj");
+}
+
+extern "C" int cuda_get_device_count(void)
+{
+    int n = 0;
+    return hipGetDeviceCount(&n) == hipSuccess ? n : 0;
+}
+
+extern "C" int cuda_set_device(int index)
+{
+    return hipSetDevice(index) == hipSuccess ? 0 : -1;
+}
+
+extern "C" int trial_div_launch_kj(const uint32_t *d_primes, uint32_t n_primes,
+                                   long long a, long long b, long long c,
+                                   uint32_t K, uint32_t J, uint32_t *d_bits, void *stream)
+{
+    return td_launch(d_primes, n_primes, a, b, c, K, J, d_bits, (hipStream_t)stream);
+}
+
+extern "C" int trial_div_launch_kj_ev(const uint32_t *d_primes, uint32_t n_primes,
+                                      long long a, long long b, long long c,
+                                      uint32_t K, uint32_t J, uint32_t *d_bits, void *stream,
+                                      void *ev_lane_start, void *ev_lane_stop)
+{
+    return td_launch(d_primes, n_primes, a, b, c, K, J, d_bits, (hipStream_t)stream,
+                     (hipEvent_t)ev_lane_start, (hipEvent_t)ev_lane_stop);
+}
+
+extern "C" int trial_div_check_error(void)
+{
+    if (hipDeviceSynchronize() != hipSuccess) return -1;
+    return td_check_error_code();
+}
+
+void trial_div_launch(const uint32_t *d_primes, uint32_t n_primes, uint64_t a, uint64_t b,
+                      uint64_t c, uint32_t *d_bits, int block_size)
+{
+    (void)block_size;
+    if (td_launch(d_primes, n_primes, (long long)a, (long long)b, (long long)c, 100u, 100u,
+                  d_bits, nullptr) != 0)
+        fprintf(stderr, "trial_div_launch: HIP error (kernel launch)\n");
+}

@@ -1,0 +1,148 @@
+"""Test-side bindings: the product library (through the package's ctypes
+module) and the CPU oracle (oracle/_build, oracle/_ref).  The oracle is only
+ever used here as the checker."""
+from __future__ import annotations
+
+import ctypes as C
+import importlib.util
+import sys
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parents[1]
+PKG = ROOT / "prime-finder_amd"
+ORACLE_LIB = ROOT / "oracle" / "_build" / "liboracle.so"
+REF_TD_LIB = ROOT / "oracle" / "_ref" / "libref_td.so"
+REF_MR_LIB = ROOT / "oracle" / "_ref" / "libref_mr.so"
+GOLDEN = ROOT / "tests" / "golden"
+
+u32p = C.POINTER(C.c_uint32)
+u8p = C.POINTER(C.c_uint8)
+ll = C.c_longlong
+
+
+def load_pf():
+    """The product package's binding module (prime-finder_amd/pf_amd.py)."""
+    if "pf_amd" in sys.modules:
+        return sys.modules["pf_amd"]
+    spec = importlib.util.spec_from_file_location("pf_amd", PKG / "pf_amd.py")
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["pf_amd"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+_oracle = None
+
+
+def oracle() -> C.CDLL:
+    global _oracle
+    if _oracle is None:
+        if not ORACLE_LIB.exists():
+            raise RuntimeError(f"{ORACLE_LIB} missing; run __graft_entry__.build()")
+        L = C.CDLL(str(ORACLE_LIB))
+        for name in ("td_oracle_sieve", "td_oracle_brute"):
+            getattr(L, name).argtypes = [u32p, C.c_uint32, ll, ll, ll, C.c_uint32, C.c_uint32, u32p]
+            getattr(L, name).restype = C.c_int
+        L.td_oracle_sieve_mt.argtypes = [u32p, C.c_uint32, ll, ll, ll, C.c_uint32, C.c_uint32,
+                                         u32p, C.c_int]
+        L.td_oracle_sieve_mt.restype = C.c_int
+        L.td_oracle_check_pairs.argtypes = [u32p, C.c_uint32, ll, ll, ll, u32p, u32p, C.c_uint32,
+                                            u8p, C.c_int]
+        L.td_oracle_check_pairs.restype = C.c_int
+        L.td_oracle_residues.argtypes = [C.c_uint32, ll, ll, ll, C.c_uint32, C.c_uint32,
+                                         C.POINTER(ll), C.POINTER(ll)]
+        L.td_oracle_residues.restype = None
+        _oracle = L
+    return _oracle
+
+
+def nwords(K: int, J: int) -> int:
+    return (K * J + 31) // 32
+
+
+def oracle_sieve(primes, a, b, c, K=100, J=100, threads: int = 0) -> np.ndarray:
+    p = np.ascontiguousarray(primes, dtype=np.uint32)
+    bits = np.zeros(nwords(K, J), np.uint32)
+    L = oracle()
+    if threads == 1:
+        rc = L.td_oracle_sieve(p.ctypes.data_as(u32p), len(p), a, b, c, K, J,
+                               bits.ctypes.data_as(u32p))
+    else:
+        rc = L.td_oracle_sieve_mt(p.ctypes.data_as(u32p), len(p), a, b, c, K, J,
+                                  bits.ctypes.data_as(u32p), threads)
+    assert rc == 0
+    return bits
+
+
+def oracle_brute(primes, a, b, c, K=100, J=100) -> np.ndarray:
+    p = np.ascontiguousarray(primes, dtype=np.uint32)
+    bits = np.zeros(nwords(K, J), np.uint32)
+    assert oracle().td_oracle_brute(p.ctypes.data_as(u32p), len(p), a, b, c, K, J,
+                                    bits.ctypes.data_as(u32p)) == 0
+    return bits
+
+
+def oracle_check_pairs(primes, a, b, c, ks, js, threads: int = 0) -> np.ndarray:
+    p = np.ascontiguousarray(primes, dtype=np.uint32)
+    ks = np.ascontiguousarray(ks, dtype=np.uint32)
+    js = np.ascontiguousarray(js, dtype=np.uint32)
+    flags = np.zeros(len(ks), np.uint8)
+    assert oracle().td_oracle_check_pairs(p.ctypes.data_as(u32p), len(p), a, b, c,
+                                          ks.ctypes.data_as(u32p), js.ctypes.data_as(u32p),
+                                          len(ks), flags.ctypes.data_as(u8p), threads) == 0
+    return flags
+
+
+class RefPrimeList(C.Structure):
+    _fields_ = [("primes", u32p), ("count", C.c_uint32)]
+
+
+_ref_td = None
+
+
+def ref_td():
+    """The reference's own trial_div_cpu (K = J = 100) built from its sources,
+    or None when oracle/_ref was not built."""
+    global _ref_td
+    if _ref_td is None and REF_TD_LIB.exists():
+        L = C.CDLL(str(REF_TD_LIB))
+        L.trial_div_cpu.argtypes = [C.POINTER(RefPrimeList), ll, ll, ll, u32p]
+        L.trial_div_cpu.restype = None
+        _ref_td = L
+    return _ref_td
+
+
+def ref_trial_div_cpu(primes, a, b, c) -> np.ndarray:
+    p = np.ascontiguousarray(primes, dtype=np.uint32)
+    bits = np.zeros(313, np.uint32)
+    pl = RefPrimeList(p.ctypes.data_as(u32p), len(p))
+    ref_td().trial_div_cpu(C.byref(pl), a, b, c, bits.ctypes.data_as(u32p))
+    return bits
+
+
+def numpy_primes(limit: int) -> np.ndarray:
+    """Independent sieve (numpy) used to cross-check the library's lists."""
+    s = np.ones(limit, dtype=bool)
+    s[:2] = False
+    for i in range(2, int(limit ** 0.5) + 1):
+        if s[i]:
+            s[i * i::i] = False
+    return np.nonzero(s)[0].astype(np.uint32)
+
+
+def bit_get(bits: np.ndarray, J: int, k: int, j: int) -> int:
+    b = (k - 1) * J + (j - 1)
+    return int((int(bits[b >> 5]) >> (b & 31)) & 1)
+
+
+def survivors(bits: np.ndarray, K: int, J: int):
+    """Sorted (k, j) survivors, lexicographic as worker.c:45-54."""
+    flat = np.unpackbits(bits.view(np.uint8), bitorder="little")[: K * J]
+    idx = np.nonzero(flat == 0)[0]
+    return [(int(i // J) + 1, int(i % J) + 1) for i in idx]
+
+
+def popcount(bits: np.ndarray) -> int:
+    return int(np.unpackbits(bits.view(np.uint8)).sum())

@@ -1,0 +1,102 @@
+"""CPU checks of the C ABI: the library loads without a GPU and exports
+every function declared in include/*.h; the host-side prime tables match an
+independent sieve and the reference's file format."""
+import ctypes as C
+import re
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+from _support import PKG, ROOT, numpy_primes
+
+INCLUDE = ROOT / "include"
+
+
+def declared_functions():
+    names = {}
+    for h in sorted(INCLUDE.glob("*.h")):
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", " ", text, flags=re.S)
+        text = re.sub(r"//[^\n]*", " ", text)
+        for m in re.finditer(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(([^;{}]*)\)\s*;", text):
+            name = m.group(1)
+            if name in ("if", "while", "for", "return", "sizeof"):
+                continue
+            names[name] = h.name
+    return names
+
+
+def exported_symbols(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", str(path)], stdout=subprocess.PIPE,
+                         text=True, check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if line.strip()}
+
+
+def test_library_exports_every_declared_function(pf):
+    lib = pf.lib()   # loads without a GPU
+    assert lib is not None
+    syms = exported_symbols(pf.LIB_PATH)
+    decl = declared_functions()
+    assert "trial_div_gpu" in decl and "cuda_set_device" in decl
+    missing = []
+    for name, hdr in decl.items():
+        if name == "trial_div_launch":
+            # C++ linkage, as in the reference (trial_div.cu:127)
+            if not any(s.startswith("_Z16trial_div_launch") for s in syms):
+                missing.append((name, hdr))
+        elif name not in syms:
+            missing.append((name, hdr))
+    assert not missing, missing
+
+
+def test_reference_abi_struct_layouts(pf):
+    assert C.sizeof(pf.PrimeList) == 16          # {uint32_t *, uint32_t} on LP64
+    assert C.sizeof(pf.GpuTiming) == 24
+
+
+def test_prime_sieve_matches_numpy(pf):
+    for limit in (2, 3, 10, 1000, 65537, 10 ** 6):
+        assert (pf.primes_below(limit) == numpy_primes(limit)).all(), limit
+    p = pf.primes_below(10 ** 6)
+    assert len(p) == 78498
+    q = pf.primes_below(10 ** 6, lo=3)
+    assert len(q) == 78497 and q[0] == 3
+
+
+@pytest.mark.slow
+def test_prime_sieve_1e8_count(pf):
+    p = pf.primes_below(10 ** 8)
+    assert len(p) == 5761455 and p[-1] == 99999989
+
+
+def _write_gap_file(path, primes):
+    """The reference's format (gen_primes.c:98-127)."""
+    out = bytearray(struct.pack("<I", len(primes)))
+    if len(primes):
+        out += struct.pack("<I", int(primes[0]))
+    for i in range(1, len(primes)):
+        gap = int(primes[i]) - int(primes[i - 1])
+        out += bytes([gap]) if gap <= 255 else bytes([0, gap & 0xFF, gap >> 8])
+    path.write_bytes(bytes(out))
+
+
+def test_prime_list_load_reference_format(pf, tmp_path):
+    L = pf.lib()
+    primes = numpy_primes(200000)[1:]                  # 2 excluded, as gen_primes writes
+    # append gaps > 255 (up to the format's u16 limit) to exercise the escape;
+    # the loader does not test primality
+    last = int(primes[-1])
+    primes = np.concatenate([primes, np.array([last + 282, last + 282 + 65535, last + 65817 + 256],
+                                              np.uint32)])
+    f = tmp_path / "p.bin"
+    _write_gap_file(f, primes)
+    pl = pf.PrimeList()
+    assert L.prime_list_load(C.byref(pl), str(f).encode()) == 0
+    got = np.ctypeslib.as_array(pl.primes, shape=(pl.count,)).copy()
+    L.prime_list_free(C.byref(pl))
+    assert (got == primes).all()
+    # truncated file -> error
+    f.write_bytes(f.read_bytes()[:-2])
+    assert L.prime_list_load(C.byref(pl), str(f).encode()) == -1
