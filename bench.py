@@ -29,9 +29,11 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 K = J = 4096
 DJ = 4
-# Algorithmic INT32 VALU ops of the lane kernel per (prime, k): (1 + D_j)
-# residues each advanced by a 3-op modular subtraction and tested by 1 compare.
-OPS_PER_PRIME_K = 4 * (1 + DJ)
+# Algorithmic INT32 VALU ops of the lane kernel per (prime, k): two residues
+# (N, and rev(N) with its j digit groups merged) each advanced by a 3-op
+# modular subtraction and tested by 1 compare.  Valid while a - c >= D_j,
+# which every triple of the workload satisfies (c = 100, a >= 50000).
+OPS_PER_PRIME_K = 4 * 2
 INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12      # 39.32 Tops/s (256 CUs, wave64, 2.4 GHz)
 SMALL_LIMIT = 1 << 16                               # primes below go to the LDS row kernel
 
@@ -75,6 +77,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one plain copy kernel of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not "
@@ -105,6 +109,15 @@ def main() -> int:
                                           None, e0, e1)
             if rc != 0:
                 raise RuntimeError("trial_div_launch_kj_ev failed")
+
+    if args.copy_mib:
+        nb = args.copy_mib << 20
+        src, dst = pf.DeviceBuffer(nb), pf.DeviceBuffer(nb)
+        src.zero()
+        assert L.pf_device_copy(dst.ptr, src.ptr, nb, None) == 0
+        assert L.pf_sync() == 0
+        src.close()
+        dst.close()
 
     for _ in range(args.warmup):
         step()
@@ -150,7 +163,7 @@ def main() -> int:
         "roofline": {"bound": "valu", "achieved": achieved, "peak": INT32_VALU_PEAK_TOPS,
                      "unit": "Tops/s", "frac": achieved / INT32_VALU_PEAK_TOPS,
                      "traffic": None,
-                     "kernel": "td_large_kernel<4>",
+                     "kernel": "td_large_kernel<4,merged>",
                      "kernel_avg_ms": avg_lane_s * 1e3,
                      "ops_per_launch": n_large * K * OPS_PER_PRIME_K},
         "survivors_first_triple": survivors0,

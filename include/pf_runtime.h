@@ -27,6 +27,9 @@ int   pf_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 int   pf_memcpy_dtod(void *dst, const void *src, size_t bytes);
 int   pf_memset(void *dst, int value, size_t bytes);
 int   pf_sync(void);
+/* Streaming device-to-device copy kernel on `stream` (bytes and both
+ * pointers 16-byte aligned); also the calibration point for HBM counters. */
+int   pf_device_copy(void *dst, const void *src, size_t bytes, void *stream);
 
 void *pf_stream_create(void);              /* default priority */
 int   pf_stream_destroy(void *stream);

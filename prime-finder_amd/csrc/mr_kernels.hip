@@ -1,0 +1,932 @@
+// mr_kernels.hip -- batched Miller-Rabin on gfx950 (kernels, host context,
+// C ABI of include/mr_gpu.h).
+//
+// Reference behaviour replaced: miller_rabin_gpu/miller_rabin_runner.cu
+// (window_exp_loop :89-203, gpu_miller_rabin_s1 :281-359, gpu_miller_rabin
+// :365-473) and batch_mod_ctx.cu (modsq_batch / modmul_batch :257-316,
+// check_passed_kernel :18-49).  Design differs: one workgroup owns one
+// candidate for a whole chunk of squarings, keeping the transform buffers in
+// LDS and every intermediate in registers, instead of ~39 separate batched
+// launches per squaring.
+//
+// Arithmetic per squaring (Montgomery, R = 2^(24 L)):
+//   T = r^2;  m = (T mod R) N' mod R;  t = (T + m N) / R;  t -= N if t >= N
+// as three transform products (r*r, T_lo*N', m*N) with NTT(N') and NTT(N)
+// precomputed.  Since T + m N == 0 (mod R), the low half of the sum is not
+// formed: t = T_hi + (mN)_hi + [T_lo != 0].
+// Exponentiation is left-to-right binary over the bits of d; a set bit
+// multiplies by the base, which is below 2^24, by double-and-add of modular
+// additions instead of a transform product.
+#include "mr_core.h"
+#include "../../include/mr_gpu.h"
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <new>
+#include <type_traits>
+#include <vector>
+
+namespace mr {
+
+// ------------------------------------------------------------------ device
+
+struct Dev {                    // per-batch device state (kernel argument)
+    int L;                      // limbs per number (R = 2^(24 L)), 2L < M
+    int stride;                 // words per linear limb vector (>= L + 1)
+    int n;                      // candidates
+    uint32_t *N;                // [n][stride] modulus limbs
+    uint32_t *Nf;               // [n][2M] NTT(N) * 2^32 per prime
+    uint32_t *Npf;              // [n][2M] NTT(N') * 2^32 per prime
+    uint32_t *one;              // [n][stride] R mod N
+    uint32_t *nm1;              // [n][stride] (N - 1) R mod N
+    uint32_t *res;              // [n][stride] current residue (Montgomery form)
+    uint32_t *scr;              // [n][2M] transform scratch
+    int32_t *nbits;             // [n]
+    int32_t *sval;              // [n] s with N - 1 = 2^s d
+    int32_t *bitpos;            // [n] next exponent bit (bit index of N), < s when done
+    uint8_t *alive;             // [n]
+    uint32_t *err;              // [1] error latch
+};
+
+struct Consts {
+    CrtConst sq, mul;           // CRT scales after Montgomery square / exact product
+    uint32_t r2[2];             // 2^64 mod p (to Montgomery form)
+    uint32_t np[2];             // -p^-1 mod 2^32
+};
+
+template <int LOGM>
+struct G {
+    static constexpr int M = 1 << LOGM;
+    static constexpr int C = M / THREADS;
+    static constexpr int NP = LOGM <= 14 ? 2 : 1;
+};
+
+template <int LOGM>
+struct Smem {
+    uint32_t X[G<LOGM>::NP * G<LOGM>::M];
+    Scratch sc;
+    uint32_t bcast[8];
+};
+
+// v[i] <- src[pos - off] for pos = t*C + i in [off, off + len), else 0.
+template <int C>
+__device__ __forceinline__ void load_vec(uint32_t (&v)[C], const uint32_t *src, int len, int off)
+{
+    const int p0 = threadIdx.x * C;
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+        const int idx = p0 + i - off;
+        v[i] = (idx >= 0 && idx < len) ? src[idx] : 0u;
+    }
+}
+
+template <int C>
+__device__ __forceinline__ void store_vec(const uint32_t (&v)[C], uint32_t *dst, int len, int off)
+{
+    const int p0 = threadIdx.x * C;
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+        const int idx = p0 + i - off;
+        if (idx >= 0 && idx < len) dst[idx] = v[i];
+    }
+}
+
+template <int C>
+__device__ __forceinline__ void zero_below(uint32_t (&v)[C], int lim)
+{
+    const int p0 = threadIdx.x * C;
+#pragma unroll
+    for (int i = 0; i < C; i++)
+        if (p0 + i < lim) v[i] = 0u;
+}
+
+// Put positions [off, off+len) of v at X_q[0..len), zero elsewhere, for the
+// NPW buffers starting at X.
+template <int LOGM, int NPW>
+__device__ __forceinline__ void stage_in(uint32_t *X, const uint32_t (&v)[G<LOGM>::C], int off, int len)
+{
+    constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
+    const int p0 = threadIdx.x * C;
+    if (off != 0) {
+#pragma unroll
+        for (int q = 0; q < NPW; q++)
+#pragma unroll
+            for (int i = 0; i < C; i++) X[q * M + p0 + i] = 0u;
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < C; i++) {
+            const int idx = p0 + i - off;
+            if (idx >= 0 && idx < len)
+#pragma unroll
+                for (int q = 0; q < NPW; q++) X[q * M + idx] = v[i];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < C; i++) {
+            const uint32_t x = (p0 + i < len) ? v[i] : 0u;
+#pragma unroll
+            for (int q = 0; q < NPW; q++) X[q * M + p0 + i] = x;
+        }
+    }
+    __syncthreads();
+}
+
+// Pointwise step on NPW buffers for primes q0..: square (B == nullptr) or
+// Montgomery product with B[(q0+q)*M + i] (an operand kept times 2^32).
+template <int LOGM, int NPW>
+__device__ __forceinline__ void pointwise(uint32_t *X, const uint32_t *B, int q0, const Consts &K)
+{
+    constexpr int M = G<LOGM>::M;
+#pragma unroll
+    for (int q = 0; q < NPW; q++) {
+        const uint32_t p = prime(q0 + q), np = K.np[q0 + q];
+        uint32_t *Y = X + q * M;
+        if (B) {
+            const uint32_t *Bq = B + (size_t)(q0 + q) * M;
+            for (int i = threadIdx.x; i < M; i += THREADS) Y[i] = mont(Y[i], Bq[i], p, np);
+        } else {
+            for (int i = threadIdx.x; i < M; i += THREADS) { const uint32_t y = Y[i]; Y[i] = mont(y, y, p, np); }
+        }
+    }
+    __syncthreads();
+}
+
+// out = A * B as normalised limbs (positions >= keep zeroed).  A is
+// positions [off, off+len) of a; B is a (square) when Bf == nullptr, else the
+// precomputed transform Bf.  a and out may alias.
+template <int LOGM>
+__device__ void tproduct(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
+                         const uint32_t (&a)[G<LOGM>::C], int off, int len,
+                         const uint32_t *Bf, uint32_t (&out)[G<LOGM>::C], int keep)
+{
+    constexpr int M = G<LOGM>::M, C = G<LOGM>::C, NP = G<LOGM>::NP;
+    const CrtConst &k = Bf ? K.mul : K.sq;
+    const int p0 = threadIdx.x * C;
+    if constexpr (NP == 2) {
+        stage_in<LOGM, 2>(sm.X, a, off, len);
+        ntt_fwd<LOGM, 2>(sm.X, T, 0);
+        pointwise<LOGM, 2>(sm.X, Bf, 0, K);
+        ntt_inv<LOGM, 2>(sm.X, T, 0);
+        const uint32_t *X0 = sm.X, *X1 = sm.X + M;
+        crt_carry<C>(out, [&](int i) { return X0[p0 + i]; }, [&](int i) { return X1[p0 + i]; },
+                     k, keep, sm.sc);
+    } else {
+        uint32_t y0[C];
+        stage_in<LOGM, 1>(sm.X, a, off, len);
+        ntt_fwd<LOGM, 1>(sm.X, T, 0);
+        pointwise<LOGM, 1>(sm.X, Bf, 0, K);
+        ntt_inv<LOGM, 1>(sm.X, T, 0);
+#pragma unroll
+        for (int i = 0; i < C; i++) y0[i] = sm.X[p0 + i];
+        __syncthreads();
+        stage_in<LOGM, 1>(sm.X, a, off, len);
+        ntt_fwd<LOGM, 1>(sm.X, T, 1);
+        pointwise<LOGM, 1>(sm.X, Bf, 1, K);
+        ntt_inv<LOGM, 1>(sm.X, T, 1);
+        const uint32_t *X1 = sm.X;
+        crt_carry<C>(out, [&](int i) { return y0[i]; }, [&](int i) { return X1[p0 + i]; },
+                     k, keep, sm.sc);
+    }
+}
+
+// dst[0..2M) = transform of positions [0, len) of v, times 2^32, per prime.
+template <int LOGM>
+__device__ void to_fft(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
+                       const uint32_t (&v)[G<LOGM>::C], int len, uint32_t *dst)
+{
+    constexpr int M = G<LOGM>::M, NP = G<LOGM>::NP;
+    for (int q0 = 0; q0 < 2; q0 += NP) {
+        stage_in<LOGM, NP>(sm.X, v, 0, len);
+        ntt_fwd<LOGM, NP>(sm.X, T, q0);
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+            const uint32_t p = prime(q0 + q), np = K.np[q0 + q], r2 = K.r2[q0 + q];
+            for (int i = threadIdx.x; i < M; i += THREADS)
+                dst[(size_t)(q0 + q) * M + i] = mont(sm.X[q * M + i], r2, p, np);
+        }
+        __syncthreads();
+    }
+}
+
+// r <- (r + s) mod N for r, s < N (high or low layout alike, N in the same
+// layout as nv).
+template <int C>
+__device__ __forceinline__ void add_modN(uint32_t (&r)[C], const uint32_t (&s)[C],
+                                         const uint32_t (&nv)[C], Scratch &sc)
+{
+    add_vec<C>(r, s, 0u, sc);
+    uint32_t d[C];
+    const bool lt = sub_vec<C>(d, r, nv, sc);
+    if (!lt) {
+#pragma unroll
+        for (int i = 0; i < C; i++) r[i] = d[i];
+    }
+}
+
+// r <- r * s mod N for r < N, 1 <= s < 2^24 (double-and-add).
+template <int C>
+__device__ void mulsmall_modN(uint32_t (&r)[C], uint32_t s, const uint32_t (&nv)[C], Scratch &sc)
+{
+    uint32_t base[C];
+#pragma unroll
+    for (int i = 0; i < C; i++) base[i] = r[i];
+    const int top = 31 - __clz((int)s);
+    for (int b = top - 1; b >= 0; b--) {
+        uint32_t cp[C];
+#pragma unroll
+        for (int i = 0; i < C; i++) cp[i] = r[i];
+        add_modN<C>(r, cp, nv, sc);
+        if ((s >> b) & 1u) add_modN<C>(r, base, nv, sc);
+    }
+}
+
+// Montgomery: r <- A * B * R^-1 mod N, output in high layout (positions
+// [L, 2L]).  A is positions [off, off+L) of r; B as in tproduct.
+template <int LOGM>
+__device__ void montmul(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, const Dev &D, int c,
+                        uint32_t (&r)[G<LOGM>::C], int off, const uint32_t *Bf)
+{
+    constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
+    const int L = D.L;
+    uint32_t t[C], m[C];
+    tproduct<LOGM>(sm, T, K, r, off, L, Bf, t, M);             // t = A*B, < R^2
+    bool nz = false;
+    const int p0 = threadIdx.x * C;
+#pragma unroll
+    for (int i = 0; i < C; i++) nz |= (p0 + i < L) && t[i] != 0u;
+    const bool cbit = block_any(nz, sm.sc);
+    const uint32_t *Npf = D.Npf + (size_t)c * 2 * M;
+    const uint32_t *Nf = D.Nf + (size_t)c * 2 * M;
+    tproduct<LOGM>(sm, T, K, t, 0, L, Npf, m, L);             // m = t_lo N' mod R
+    tproduct<LOGM>(sm, T, K, m, 0, L, Nf, m, M);              // m = m N
+    zero_below<C>(t, L);
+    zero_below<C>(m, L);
+    add_vec<C>(t, m, 0u, sm.sc);                              // T_hi + (mN)_hi, shifted by L
+    // + [T_lo != 0] at position L
+    {
+        uint32_t one[C];
+#pragma unroll
+        for (int i = 0; i < C; i++) one[i] = (cbit && p0 + i == L) ? 1u : 0u;
+        add_vec<C>(t, one, 0u, sm.sc);
+    }
+    load_vec<C>(m, D.N + (size_t)c * D.stride, L, L);         // N, high layout
+    const bool lt = sub_vec<C>(r, t, m, sm.sc);
+    if (lt) {
+#pragma unroll
+        for (int i = 0; i < C; i++) r[i] = t[i];
+    }
+}
+
+// ------------------------------------------------------------------ kernels
+
+__global__ void twiddle_kernel(uint32_t *out, int logm, uint32_t w0, uint32_t w1)
+{
+    const int half = 1 << (logm - 1);
+    const int M = 1 << logm;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= half) return;
+    for (int q = 0; q < 2; q++) {
+        const uint32_t p = prime(q);
+        const uint64_t w = q ? w1 : w0;
+        auto pw = [&](uint64_t e) {
+            uint64_t acc = 1, b = w;
+            while (e) { if (e & 1) acc = acc * b % p; b = b * b % p; e >>= 1; }
+            return (uint32_t)acc;
+        };
+        const uint32_t f = pw((uint64_t)i), g = pw((uint64_t)(M - i) % (uint64_t)M);
+        uint32_t *base = out + (size_t)q * 4 * half;
+        base[i] = f;
+        base[half + i] = (uint32_t)(((uint64_t)f << 32) / p);
+        base[2 * half + i] = g;
+        base[3 * half + i] = (uint32_t)(((uint64_t)g << 32) / p);
+    }
+}
+
+// words (uint32, n_words per number) -> 24-bit limbs (stride per number).
+__global__ void words_to_limbs(const uint32_t *w, int n_words, uint32_t *limbs, int stride, int L)
+{
+    const int c = blockIdx.y;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < stride; i += gridDim.x * blockDim.x) {
+        uint32_t v = 0;
+        if (i < L) {
+            const long bit = (long)i * 24;
+            const long wi = bit >> 5;
+            const int sh = (int)(bit & 31);
+            uint64_t x = 0;
+            if (wi < n_words) x = w[(size_t)c * n_words + wi];
+            if (wi + 1 < n_words) x |= (uint64_t)w[(size_t)c * n_words + wi + 1] << 32;
+            v = (uint32_t)(x >> sh) & LIMB_MASK;
+        }
+        limbs[(size_t)c * stride + i] = v;
+    }
+}
+
+__global__ void limbs_to_words(const uint32_t *limbs, int stride, int L, uint32_t *w, int n_words,
+                               size_t w_stride)
+{
+    const int c = blockIdx.y;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n_words; i += gridDim.x * blockDim.x) {
+        const long bit = (long)i * 32;
+        const long li = bit / 24;
+        const int sh = (int)(bit % 24);
+        uint64_t x = 0;
+        for (int k = 0; k < 3; k++)
+            if (li + k < L) x |= (uint64_t)limbs[(size_t)c * stride + li + k] << (24 * k);
+        w[(size_t)c * w_stride + i] = (uint32_t)(x >> sh);
+    }
+}
+
+// Per-candidate constants: nbits, s, NTT(N), N' (Newton), NTT(N'), R mod N,
+// (N-1)R mod N.  One workgroup per candidate.
+template <int LOGM>
+__global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Consts K)
+{
+    constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
+    __shared__ Smem<LOGM> sm;
+    const int c = blockIdx.x;
+    const int L = D.L;
+    const uint32_t *Nl = D.N + (size_t)c * D.stride;
+    uint32_t nv[C], y[C], e[C];
+    load_vec<C>(nv, Nl, L, 0);
+
+    // bit length and two-adic valuation of N - 1
+    {
+        int top = -1, low = 1 << 30;
+        const int p0 = threadIdx.x * C;
+#pragma unroll
+        for (int i = 0; i < C; i++) {
+            if (nv[i]) top = max(top, (p0 + i) * 24 + 31 - __clz((int)nv[i]));
+            const uint32_t x = (p0 + i == 0) ? (nv[i] & ~1u) : nv[i];
+            if (x) low = min(low, (p0 + i) * 24 + __ffs((int)x) - 1);
+        }
+        if (threadIdx.x == 0) { sm.bcast[0] = 0; sm.bcast[1] = 0x7FFFFFFF; }
+        __syncthreads();
+        if (top >= 0) atomicMax(&sm.bcast[0], (uint32_t)(top + 1));
+        if (low < (1 << 30)) atomicMin(&sm.bcast[1], (uint32_t)low);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            D.nbits[c] = (int32_t)sm.bcast[0];
+            D.sval[c] = (int32_t)sm.bcast[1];
+            if (!(Nl[0] & 1u) || sm.bcast[0] < 3 || (int)sm.bcast[0] > 24 * L - 1)
+                atomicOr(D.err, 1u);
+        }
+        __syncthreads();
+    }
+    const int nbits = (int)sm.bcast[0];
+
+    uint32_t *Nf = D.Nf + (size_t)c * 2 * M;
+    uint32_t *Npf = D.Npf + (size_t)c * 2 * M;
+    uint32_t *scr = D.scr + (size_t)c * 2 * M;
+    to_fft<LOGM>(sm, T, K, nv, L, Nf);
+
+    // N' = -N^-1 mod R by Newton: y <- y (2 + N y), doubling precision.
+    {
+        uint32_t n0 = Nl[0];
+        uint32_t inv = n0;
+        for (int i = 0; i < 5; i++) inv *= 2u - n0 * inv;
+#pragma unroll
+        for (int i = 0; i < C; i++) y[i] = 0u;
+        if (threadIdx.x == 0) y[0] = (0u - inv) & LIMB_MASK;
+        for (int prec = 1; prec < L; prec *= 2) {
+            tproduct<LOGM>(sm, T, K, y, 0, L, Nf, e, L);     // e = N y mod R
+            uint32_t two[C];
+#pragma unroll
+            for (int i = 0; i < C; i++) two[i] = (threadIdx.x == 0 && i == 0) ? 2u : 0u;
+            add_vec<C>(e, two, 0u, sm.sc);
+            {   // truncate to R
+                const int p0 = threadIdx.x * C;
+#pragma unroll
+                for (int i = 0; i < C; i++) if (p0 + i >= L) e[i] = 0u;
+            }
+            to_fft<LOGM>(sm, T, K, e, L, scr);
+            tproduct<LOGM>(sm, T, K, y, 0, L, scr, y, L);    // y = y (2 + N y) mod R
+        }
+    }
+    to_fft<LOGM>(sm, T, K, y, L, Npf);
+
+    // R mod N: 2^nbits - N, then doubled (24 L - nbits) times mod N.
+    {
+        uint32_t pw[C];
+        const int p0 = threadIdx.x * C;
+#pragma unroll
+        for (int i = 0; i < C; i++)
+            pw[i] = (p0 + i == nbits / 24) ? (1u << (nbits % 24)) : 0u;
+        sub_vec<C>(y, pw, nv, sm.sc);                  // 2^nbits - N  (< N)
+        for (int k = nbits; k < 24 * L; k++) {
+            uint32_t cp[C];
+#pragma unroll
+            for (int i = 0; i < C; i++) cp[i] = y[i];
+            add_modN<C>(y, cp, nv, sm.sc);
+        }
+        store_vec<C>(y, D.one + (size_t)c * D.stride, D.stride, 0);
+        sub_vec<C>(e, nv, y, sm.sc);                   // N - (R mod N) = (N-1) R mod N
+        store_vec<C>(e, D.nm1 + (size_t)c * D.stride, D.stride, 0);
+    }
+}
+
+// Start a round: r = base * R mod N, bitpos = nbits - 2 (the top bit of d
+// is 1 and is consumed by the initial value).
+template <int LOGM>
+__global__ __launch_bounds__(THREADS) void round_start_kernel(Dev D, uint32_t base)
+{
+    constexpr int C = G<LOGM>::C;
+    __shared__ Scratch sc;
+    const int c = blockIdx.x;
+    if (!D.alive[c]) return;
+    uint32_t r[C], nv[C];
+    load_vec<C>(nv, D.N + (size_t)c * D.stride, D.L, 0);
+    load_vec<C>(r, D.one + (size_t)c * D.stride, D.L, 0);
+    if (base > 1u) mulsmall_modN<C>(r, base, nv, sc);
+    store_vec<C>(r, D.res + (size_t)c * D.stride, D.stride, 0);
+    if (threadIdx.x == 0) D.bitpos[c] = D.nbits[c] - 2;
+}
+
+// Up to `steps` exponent bits: r = r^2, times base when the bit of d is set.
+// Bit i of d is bit i + s of N (N odd, so N - 1 differs only in bit 0).
+template <int LOGM>
+__global__ __launch_bounds__(THREADS) void exp_kernel(Dev D, Twiddles T, Consts K, uint32_t base,
+                                                      int steps)
+{
+    constexpr int C = G<LOGM>::C;
+    __shared__ Smem<LOGM> sm;
+    const int c = blockIdx.x;
+    if (!D.alive[c]) return;
+    const int s = D.sval[c];
+    int bp = D.bitpos[c];
+    if (bp < s) return;
+    const int L = D.L;
+    const uint32_t *Nl = D.N + (size_t)c * D.stride;
+    uint32_t r[C], nv[C];
+    load_vec<C>(r, D.res + (size_t)c * D.stride, L, L);       // high layout
+    load_vec<C>(nv, Nl, L, L);
+    for (int k = 0; k < steps && bp >= s; k++, bp--) {
+        montmul<LOGM>(sm, T, K, D, c, r, L, nullptr);
+        const uint32_t bit = (Nl[bp / 24] >> (bp % 24)) & 1u;
+        if (bit) mulsmall_modN<C>(r, base, nv, sm.sc);
+    }
+    store_vec<C>(r, D.res + (size_t)c * D.stride, L, L);
+    if (threadIdx.x == 0) D.bitpos[c] = bp;
+}
+
+template <int C>
+__device__ __forceinline__ bool vec_equal(const uint32_t (&a)[C], const uint32_t (&b)[C], Scratch &sc)
+{
+    bool ne = false;
+#pragma unroll
+    for (int i = 0; i < C; i++) ne |= a[i] != b[i];
+    return !block_any(ne, sc);
+}
+
+// End of a round: pass iff r == R or r^(2^i) == (N-1) R for some i < s;
+// optionally writes r out of Montgomery form (a^d mod N) to out.
+template <int LOGM>
+__global__ __launch_bounds__(THREADS) void round_end_kernel(Dev D, Twiddles T, Consts K,
+                                                            uint32_t *out, size_t out_stride)
+{
+    constexpr int C = G<LOGM>::C;
+    __shared__ Smem<LOGM> sm;
+    const int c = blockIdx.x;
+    if (!D.alive[c]) return;
+    const int L = D.L;
+    uint32_t r[C], a[C], b[C];
+    load_vec<C>(r, D.res + (size_t)c * D.stride, L, L);
+    load_vec<C>(a, D.one + (size_t)c * D.stride, L, L);
+    load_vec<C>(b, D.nm1 + (size_t)c * D.stride, L, L);
+    bool pass = vec_equal<C>(r, a, sm.sc) || vec_equal<C>(r, b, sm.sc);
+    if (out) {   // a^d mod N = REDC(r): product with 1
+        uint32_t x[C];
+#pragma unroll
+        for (int i = 0; i < C; i++) x[i] = r[i];
+        uint32_t *one_f = D.scr + (size_t)c * 2 * G<LOGM>::M;
+        uint32_t unit[C];
+#pragma unroll
+        for (int i = 0; i < C; i++) unit[i] = (threadIdx.x == 0 && i == 0) ? 1u : 0u;
+        to_fft<LOGM>(sm, T, K, unit, L, one_f);
+        montmul<LOGM>(sm, T, K, D, c, x, L, one_f);
+        store_vec<C>(x, out + (size_t)c * out_stride, L, L);
+    }
+    const int s = D.sval[c];
+    for (int i = 1; i < s && !pass; i++) {
+        montmul<LOGM>(sm, T, K, D, c, r, L, nullptr);
+        pass = vec_equal<C>(r, b, sm.sc);
+        if (!pass && vec_equal<C>(r, a, sm.sc)) break;     // reached 1: composite
+    }
+    if (threadIdx.x == 0 && !pass) D.alive[c] = 0;
+}
+
+// Self-check kernels: op 0 product a*b (2L limbs), op 1 a*b*R^-1 mod N,
+// op 2 a*s mod N.  a, b, out: [n][stride2] limbs with stride2 >= 2L+1.
+template <int LOGM>
+__global__ __launch_bounds__(THREADS) void debug_kernel(Dev D, Twiddles T, Consts K, int op,
+                                                        const uint32_t *A, const uint32_t *B,
+                                                        uint32_t s, uint32_t *out, int stride2)
+{
+    constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
+    __shared__ Smem<LOGM> sm;
+    const int c = blockIdx.x;
+    const int L = D.L;
+    uint32_t a[C], b[C];
+    load_vec<C>(a, A + (size_t)c * stride2, L, 0);
+    load_vec<C>(b, B + (size_t)c * stride2, L, 0);
+    uint32_t *scr = D.scr + (size_t)c * 2 * M;
+    if (op == 0) {
+        to_fft<LOGM>(sm, T, K, b, L, scr);
+        tproduct<LOGM>(sm, T, K, a, 0, L, scr, a, M);
+        store_vec<C>(a, out + (size_t)c * stride2, 2 * L, 0);
+    } else if (op == 1) {
+        to_fft<LOGM>(sm, T, K, b, L, scr);
+        montmul<LOGM>(sm, T, K, D, c, a, 0, scr);
+        store_vec<C>(a, out + (size_t)c * stride2, L, L);
+    } else {
+        uint32_t nv[C];
+        load_vec<C>(nv, D.N + (size_t)c * D.stride, L, 0);
+        mulsmall_modN<C>(a, s, nv, sm.sc);
+        store_vec<C>(a, out + (size_t)c * stride2, L, 0);
+    }
+}
+
+} // namespace mr
+
+// ------------------------------------------------------------------ host
+
+namespace {
+
+using namespace mr;
+
+uint64_t h_pow(uint64_t b, uint64_t e, uint64_t p)
+{
+    uint64_t acc = 1;
+    b %= p;
+    while (e) {
+        if (e & 1) acc = (unsigned __int128)acc * b % p;
+        b = (unsigned __int128)b * b % p;
+        e >>= 1;
+    }
+    return acc;
+}
+
+uint32_t h_shoupq(uint32_t w, uint32_t p) { return (uint32_t)(((uint64_t)w << 32) / p); }
+
+Consts make_consts(int logm)
+{
+    Consts K;
+    const uint64_t M = 1ull << logm;
+    const uint32_t minv0 = (uint32_t)h_pow(M, P0 - 2, P0), minv1 = (uint32_t)h_pow(M, P1 - 2, P1);
+    const uint32_t r32_0 = (uint32_t)((1ull << 32) % P0), r32_1 = (uint32_t)((1ull << 32) % P1);
+    auto fill = [&](CrtConst &k, uint32_t s0, uint32_t s1) {
+        k.s0 = s0; k.s0q = h_shoupq(s0, P0);
+        k.c1 = (uint32_t)((uint64_t)s1 * P0_INV_MOD_P1 % P1); k.c1q = h_shoupq(k.c1, P1);
+        k.c2 = P0_INV_MOD_P1; k.c2q = h_shoupq(k.c2, P1);
+    };
+    fill(K.mul, minv0, minv1);
+    fill(K.sq, (uint32_t)((uint64_t)minv0 * r32_0 % P0), (uint32_t)((uint64_t)minv1 * r32_1 % P1));
+    K.r2[0] = (uint32_t)((uint64_t)r32_0 * r32_0 % P0);
+    K.r2[1] = (uint32_t)((uint64_t)r32_1 * r32_1 % P1);
+    K.np[0] = neg_inv32(P0);
+    K.np[1] = neg_inv32(P1);
+    return K;
+}
+
+constexpr int MIN_LOGM = 11, MAX_LOGM = 14;
+
+struct TwCache {
+    uint32_t *tw[MAX_LOGM + 1] = {};
+};
+TwCache g_tw[64];
+
+int get_twiddles(int dev, int logm, hipStream_t st, uint32_t **out)
+{
+    if (dev < 0 || dev >= 64) return -1;
+    uint32_t *&t = g_tw[dev].tw[logm];
+    if (!t) {
+        const int half = 1 << (logm - 1);
+        if (hipMalloc(&t, (size_t)2 * 4 * half * sizeof(uint32_t)) != hipSuccess) { t = nullptr; return -1; }
+        const uint32_t w0 = (uint32_t)h_pow(G0, (P0 - 1) >> logm, P0);
+        const uint32_t w1 = (uint32_t)h_pow(G1, (P1 - 1) >> logm, P1);
+        twiddle_kernel<<<(half + 255) / 256, 256, 0, st>>>(t, logm, w0, w1);
+        if (hipStreamSynchronize(st) != hipSuccess) return -1;
+    }
+    *out = t;
+    return 0;
+}
+
+} // namespace
+
+struct mr_ctx {
+    int device = 0;
+    int n = 0, n_words = 0, L = 0, logm = 0, stride = 0;
+    hipStream_t st = nullptr;
+    Dev d{};
+    Consts K{};
+    Twiddles T{};
+    std::vector<void *> allocs;
+    std::vector<int32_t> sval;
+};
+
+namespace {
+
+const char *kWho = "mr_gpu";
+
+int hip_fail(hipError_t e, const char *what)
+{
+    fprintf(stderr, "%s: HIP error %s (%s)\n", kWho, hipGetErrorName(e), what);
+    return -2;
+}
+
+#define MR_CHECK(call)                                        \
+    do {                                                      \
+        hipError_t e_ = (call);                               \
+        if (e_ != hipSuccess) return hip_fail(e_, #call);     \
+    } while (0)
+
+template <class F>
+int dispatch_logm(int logm, F f)
+{
+    switch (logm) {
+    case 11: return f(std::integral_constant<int, 11>{});
+    case 12: return f(std::integral_constant<int, 12>{});
+    case 13: return f(std::integral_constant<int, 13>{});
+    case 14: return f(std::integral_constant<int, 14>{});
+    default: return -1;
+    }
+}
+
+int ctx_alloc(mr_ctx *x, void **p, size_t bytes)
+{
+    MR_CHECK(hipMalloc(p, bytes ? bytes : 4));
+    x->allocs.push_back(*p);
+    MR_CHECK(hipMemsetAsync(*p, 0, bytes ? bytes : 4, x->st));
+    return 0;
+}
+
+// Common construction once N words are resident at dN_words ([n][n_words]).
+int ctx_init(mr_ctx *x, const uint32_t *h_words)
+{
+    const int bits = x->n_words * 32;
+    int L = (bits + 23) / 24;
+    if (L < 4) L = 4;
+    int logm = MIN_LOGM;
+    while ((1 << logm) < 2 * L + 2) logm++;
+    if (logm > MAX_LOGM || L > MAX_LIMBS) {
+        fprintf(stderr, "%s: modulus of %d bits is beyond this build's transform sizes\n", kWho, bits);
+        return -1;
+    }
+    x->L = L;
+    x->logm = logm;
+    x->stride = (1 << logm);   // room for 2L+1 limbs in debug products
+    const int M = 1 << logm;
+    const size_t n = (size_t)x->n;
+    Dev &d = x->d;
+    d.L = L;
+    d.stride = x->stride;
+    d.n = x->n;
+    void *p;
+#define MR_ALLOC(field, type, count)                                   \
+    if (ctx_alloc(x, &p, sizeof(type) * (size_t)(count)) != 0) return -2; \
+    d.field = (type *)p;
+    MR_ALLOC(N, uint32_t, n * x->stride);
+    MR_ALLOC(Nf, uint32_t, n * 2 * M);
+    MR_ALLOC(Npf, uint32_t, n * 2 * M);
+    MR_ALLOC(one, uint32_t, n * x->stride);
+    MR_ALLOC(nm1, uint32_t, n * x->stride);
+    MR_ALLOC(res, uint32_t, n * x->stride);
+    MR_ALLOC(scr, uint32_t, n * 2 * M);
+    MR_ALLOC(nbits, int32_t, n);
+    MR_ALLOC(sval, int32_t, n);
+    MR_ALLOC(bitpos, int32_t, n);
+    MR_ALLOC(alive, uint8_t, n);
+    MR_ALLOC(err, uint32_t, 1);
+#undef MR_ALLOC
+    uint32_t *dw = nullptr;
+    if (ctx_alloc(x, (void **)&dw, sizeof(uint32_t) * n * x->n_words) != 0) return -2;
+    MR_CHECK(hipMemcpyAsync(dw, h_words, sizeof(uint32_t) * n * x->n_words, hipMemcpyHostToDevice, x->st));
+    words_to_limbs<<<dim3(8, x->n), 256, 0, x->st>>>(dw, x->n_words, d.N, x->stride, L);
+    uint32_t *tw = nullptr;
+    if (get_twiddles(x->device, logm, x->st, &tw) != 0) return hip_fail(hipErrorUnknown, "twiddles");
+    x->T.base = tw;
+    x->T.half = M / 2;
+    x->K = make_consts(logm);
+    int rc = dispatch_logm(logm, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        setup_kernel<LG><<<x->n, THREADS, 0, x->st>>>(x->d, x->T, x->K);
+        return 0;
+    });
+    if (rc) return rc;
+    MR_CHECK(hipGetLastError());
+    uint32_t err = 0;
+    MR_CHECK(hipMemcpyAsync(&err, d.err, 4, hipMemcpyDeviceToHost, x->st));
+    x->sval.resize(n);
+    MR_CHECK(hipMemcpyAsync(x->sval.data(), d.sval, 4 * n, hipMemcpyDeviceToHost, x->st));
+    MR_CHECK(hipStreamSynchronize(x->st));
+    if (err) {
+        fprintf(stderr, "%s: every modulus must be odd, > 3 and fit the context\n", kWho);
+        return -1;
+    }
+    return 0;
+}
+
+mr_ctx *ctx_new(int n_words, int n_batch, int device)
+{
+    if (n_words < 1 || n_batch < 1) return nullptr;
+    mr_ctx *x = new (std::nothrow) mr_ctx;
+    if (!x) return nullptr;
+    x->device = device;
+    x->n = n_batch;
+    x->n_words = n_words;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&x->st, hipStreamNonBlocking) != hipSuccess) {
+        fprintf(stderr, "%s: cannot use HIP device %d\n", kWho, device);
+        delete x;
+        return nullptr;
+    }
+    return x;
+}
+
+} // namespace
+
+extern "C" void mr_ctx_destroy(mr_ctx_t *x)
+{
+    if (!x) return;
+    (void)hipSetDevice(x->device);
+    if (x->st) (void)hipStreamSynchronize(x->st);
+    for (void *p : x->allocs) (void)hipFree(p);
+    if (x->st) (void)hipStreamDestroy(x->st);
+    delete x;
+}
+
+extern "C" mr_ctx_t *mr_ctx_create(const uint32_t *N_words, int n_words, int n_batch, int device)
+{
+    if (!N_words) return nullptr;
+    mr_ctx *x = ctx_new(n_words, n_batch, device);
+    if (!x) return nullptr;
+    if (ctx_init(x, N_words) != 0) {
+        mr_ctx_destroy(x);
+        return nullptr;
+    }
+    return x;
+}
+
+extern "C" mr_ctx_t *mr_ctx_create_limbs16(const uint64_t *N_all, int n_limbs16, int n_batch, int device)
+{
+    if (!N_all || n_limbs16 < 1 || n_batch < 1) return nullptr;
+    const int n_words = (n_limbs16 + 1) / 2;
+    std::vector<uint32_t> w((size_t)n_batch * n_words, 0u);
+    for (int c = 0; c < n_batch; c++)
+        for (int i = 0; i < n_limbs16; i++) {
+            const uint64_t v = N_all[(size_t)c * n_limbs16 + i];
+            if (v > 0xFFFFu) return nullptr;
+            w[(size_t)c * n_words + i / 2] |= (uint32_t)v << (16 * (i & 1));
+        }
+    return mr_ctx_create(w.data(), n_words, n_batch, device);
+}
+
+extern "C" int mr_ctx_info(const mr_ctx_t *x, int *n_words, int *limbs, int *log_len, int *s_out)
+{
+    if (!x) return -1;
+    if (n_words) *n_words = x->n_words;
+    if (limbs) *limbs = x->L;
+    if (log_len) *log_len = x->logm;
+    if (s_out) memcpy(s_out, x->sval.data(), sizeof(int32_t) * x->n);
+    return 0;
+}
+
+extern "C" int mr_ctx_moduli(const mr_ctx_t *x, uint32_t *out)
+{
+    if (!x || !out) return -1;
+    (void)hipSetDevice(x->device);
+    uint32_t *dw = nullptr;
+    const size_t bytes = sizeof(uint32_t) * (size_t)x->n * x->n_words;
+    MR_CHECK(hipMalloc(&dw, bytes));
+    limbs_to_words<<<dim3(8, x->n), 256, 0, x->st>>>(x->d.N, x->stride, x->L, dw, x->n_words, x->n_words);
+    hipError_t e = hipMemcpyAsync(out, dw, bytes, hipMemcpyDeviceToHost, x->st);
+    if (e == hipSuccess) e = hipStreamSynchronize(x->st);
+    (void)hipFree(dw);
+    return e == hipSuccess ? 0 : hip_fail(e, "mr_ctx_moduli");
+}
+
+extern "C" int mr_run(mr_ctx_t *x, const uint32_t *bases, int n_bases, uint8_t *alive_out,
+                      uint32_t *residues_out)
+{
+    if (!x || !bases || n_bases < 0 || !alive_out) return -1;
+    for (int i = 0; i < n_bases; i++)
+        if (bases[i] < 2u || bases[i] >= (1u << 24)) return -1;
+    MR_CHECK(hipSetDevice(x->device));
+    const int n = x->n;
+    MR_CHECK(hipMemsetAsync(x->d.alive, 1, n, x->st));
+    uint32_t *dres = nullptr;
+    uint32_t *dlimbs = nullptr;
+    const size_t rbytes = sizeof(uint32_t) * (size_t)n * n_bases * x->n_words;
+    if (residues_out) {
+        MR_CHECK(hipMalloc(&dres, rbytes ? rbytes : 4));
+        MR_CHECK(hipMemsetAsync(dres, 0, rbytes ? rbytes : 4, x->st));
+        MR_CHECK(hipMalloc(&dlimbs, sizeof(uint32_t) * (size_t)n * x->stride));
+    }
+    int max_bits = 0;
+    {
+        std::vector<int32_t> nb(n);
+        MR_CHECK(hipMemcpyAsync(nb.data(), x->d.nbits, 4 * n, hipMemcpyDeviceToHost, x->st));
+        MR_CHECK(hipStreamSynchronize(x->st));
+        for (int v : nb) max_bits = v > max_bits ? v : max_bits;
+    }
+    const int CHUNK = 256;
+    int rc = 0;
+    for (int bi = 0; bi < n_bases && rc == 0; bi++) {
+        const uint32_t base = bases[bi];
+        rc = dispatch_logm(x->logm, [&](auto lg) {
+            constexpr int LG = decltype(lg)::value;
+            round_start_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, base);
+            for (int done = 0; done < max_bits; done += CHUNK)
+                exp_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, x->T, x->K, base, CHUNK);
+            if (residues_out) {
+                MR_CHECK(hipMemsetAsync(dlimbs, 0, sizeof(uint32_t) * (size_t)n * x->stride, x->st));
+                round_end_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, x->T, x->K, dlimbs, x->stride);
+                limbs_to_words<<<dim3(8, n), 256, 0, x->st>>>(
+                    dlimbs, x->stride, x->L, dres + (size_t)bi * x->n_words, x->n_words,
+                    (size_t)n_bases * x->n_words);
+            } else {
+                round_end_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, x->T, x->K, nullptr, 0);
+            }
+            return 0;
+        });
+        if (rc == 0) {
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) rc = hip_fail(e, "mr_run launch");
+        }
+    }
+    if (rc == 0) {
+        hipError_t e = hipMemcpyAsync(alive_out, x->d.alive, n, hipMemcpyDeviceToHost, x->st);
+        if (e == hipSuccess && residues_out)
+            e = hipMemcpyAsync(residues_out, dres, rbytes, hipMemcpyDeviceToHost, x->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(x->st);
+        if (e != hipSuccess) rc = hip_fail(e, "mr_run");
+    }
+    if (dres) (void)hipFree(dres);
+    if (dlimbs) (void)hipFree(dlimbs);
+    return rc;
+}
+
+namespace {
+
+int debug_op(mr_ctx *x, int op, const uint32_t *a, const uint32_t *b, uint32_t s, uint32_t *out)
+{
+    if (!x || !a || !out || (op != 2 && !b)) return -1;
+    MR_CHECK(hipSetDevice(x->device));
+    const int n = x->n, nw = x->n_words;
+    const size_t wb = sizeof(uint32_t) * (size_t)n * nw;
+    const int out_words = op == 0 ? 2 * nw : nw;
+    uint32_t *da = nullptr, *db = nullptr, *la = nullptr, *lb = nullptr, *lo = nullptr, *dout = nullptr;
+    const size_t lb_bytes = sizeof(uint32_t) * (size_t)n * x->stride;
+    int rc = 0;
+    hipError_t e = hipSuccess;
+    if ((e = hipMalloc(&da, wb)) != hipSuccess || (e = hipMalloc(&db, wb)) != hipSuccess ||
+        (e = hipMalloc(&la, lb_bytes)) != hipSuccess || (e = hipMalloc(&lb, lb_bytes)) != hipSuccess ||
+        (e = hipMalloc(&lo, lb_bytes)) != hipSuccess ||
+        (e = hipMalloc(&dout, sizeof(uint32_t) * (size_t)n * out_words)) != hipSuccess) {
+        rc = hip_fail(e, "debug alloc");
+    }
+    if (rc == 0) {
+        (void)hipMemcpyAsync(da, a, wb, hipMemcpyHostToDevice, x->st);
+        (void)hipMemcpyAsync(db, op == 2 ? a : b, wb, hipMemcpyHostToDevice, x->st);
+        (void)hipMemsetAsync(lo, 0, lb_bytes, x->st);
+        words_to_limbs<<<dim3(8, n), 256, 0, x->st>>>(da, nw, la, x->stride, x->L);
+        words_to_limbs<<<dim3(8, n), 256, 0, x->st>>>(db, nw, lb, x->stride, x->L);
+        rc = dispatch_logm(x->logm, [&](auto lg) {
+            constexpr int LG = decltype(lg)::value;
+            debug_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, x->T, x->K, op, la, lb, s, lo, x->stride);
+            return 0;
+        });
+        limbs_to_words<<<dim3(8, n), 256, 0, x->st>>>(lo, x->stride, op == 0 ? 2 * x->L : x->L,
+                                                        dout, out_words, out_words);
+        e = hipMemcpyAsync(out, dout, sizeof(uint32_t) * (size_t)n * out_words, hipMemcpyDeviceToHost, x->st);
+        if (e == hipSuccess) e = hipStreamSynchronize(x->st);
+        if (e != hipSuccess) rc = hip_fail(e, "debug op");
+    }
+    (void)hipFree(da); (void)hipFree(db); (void)hipFree(la); (void)hipFree(lb); (void)hipFree(lo);
+    (void)hipFree(dout);
+    return rc;
+}
+
+} // namespace
+
+extern "C" int mr_debug_product(mr_ctx_t *x, const uint32_t *a, const uint32_t *b, uint32_t *out)
+{
+    return debug_op(x, 0, a, b, 0, out);
+}
+
+extern "C" int mr_debug_montmul(mr_ctx_t *x, const uint32_t *a, const uint32_t *b, uint32_t *out)
+{
+    return debug_op(x, 1, a, b, 0, out);
+}
+
+extern "C" int mr_debug_mulsmall(mr_ctx_t *x, const uint32_t *a, uint32_t s, uint32_t *out)
+{
+    if (s < 1u || s >= (1u << 24)) return -1;
+    return debug_op(x, 2, a, nullptr, s, out);
+}
+
+extern "C" mr_ctx_t *mr_ctx_create_sparse(const mr_candidate_t *cands, int n_batch, int device)
+{
+    (void)cands; (void)n_batch; (void)device;
+    fprintf(stderr, "%s: mr_ctx_create_sparse: device candidate builder not built yet\n", kWho);
+    return nullptr;
+}

@@ -1,6 +1,7 @@
 // pf_runtime.hip -- thin C-ABI wrappers over the HIP runtime (see
 // include/pf_runtime.h).
 #include <hip/hip_runtime.h>
+#include <stdint.h>
 #include <string.h>
 
 #include "../../include/pf_runtime.h"
@@ -75,6 +76,28 @@ extern "C" int pf_memset(void *dst, int value, size_t bytes)
 }
 
 extern "C" int pf_sync(void) { return pf_ok(hipDeviceSynchronize()); }
+
+// A plain streaming copy (16-byte vectors, grid-stride): the reference point
+// for HBM byte counters, and a device-to-device copy usable on any stream.
+__global__ __launch_bounds__(256) void pf_copy_kernel(uint4 *__restrict__ dst,
+                                                     const uint4 *__restrict__ src, size_t n16)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n16;
+         i += (size_t)gridDim.x * 256)
+        dst[i] = src[i];
+}
+
+extern "C" int pf_device_copy(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if ((bytes & 15u) || ((uintptr_t)dst & 15u) || ((uintptr_t)src & 15u)) return -1;
+    const size_t n16 = bytes / 16u;
+    if (n16 == 0) return 0;
+    size_t blocks = (n16 + 255u) / 256u;
+    if (blocks > 8192u) blocks = 8192u;
+    pf_copy_kernel<<<(unsigned)blocks, 256, 0, (hipStream_t)stream>>>(
+        (uint4 *)dst, (const uint4 *)src, n16);
+    return pf_ok(hipGetLastError());
+}
 
 extern "C" void *pf_stream_create(void)
 {

@@ -27,7 +27,10 @@
 //                          band is OR-ed into HBM once;
 //   td_large_kernel        one lane per prime >= 2^16: Montgomery setup, then
 //                          a k loop of 3-op modular steps with a rare
-//                          divergent hit path (global atomic OR).
+//                          divergent hit path (global atomic OR).  The rev
+//                          condition's j digit groups share one residue when
+//                          a-c >= D_j (see td_hits), so a k step costs two
+//                          residue updates and two compares.
 #include "pf_hip.h"
 #include "../../include/trial_div.h"
 
@@ -38,13 +41,14 @@ namespace {
 
 constexpr uint32_t TD_SMALL_LIMIT = 1u << 16;   // row kernel below, lane kernel above
 constexpr uint32_t TD_SMALL_CAP   = 65536;      // list entries below TD_SMALL_LIMIT
-constexpr uint32_t TD_TBL         = 32;         // uint32 per small-prime table entry
-constexpr uint32_t TD_ROWS        = 16;         // rows per row-kernel workgroup
+constexpr uint32_t TD_TBL         = 25;         // uint32 fields per small-prime table entry
+constexpr uint32_t TD_ROWS        = 4;          // rows per row-kernel workgroup
 constexpr uint32_t TD_BLOCK       = 256;
 constexpr uint32_t TD_PMAX        = 0x80000000u;
 constexpr int      TD_SETUP_GRID  = 32;
 
-// Small-prime table entry (TD_TBL words):
+// Small-prime table, structure of arrays: field f of entry i lives at
+// tbl[f * TD_SMALL_CAP + i], so lanes reading consecutive entries coalesce.
 enum {
     TT_P = 0, TT_MAGIC = 1, TT_FLAGS = 2, TT_UN0 = 3, TT_B = 4,
     TT_CM = 5,   // 4 words: Cm[lj-1]
@@ -142,7 +146,7 @@ void td_small_setup_kernel(const uint32_t *__restrict__ small,
     for (uint32_t i = blockIdx.x * TD_BLOCK + threadIdx.x; i < count;
          i += gridDim.x * TD_BLOCK) {
         const uint32_t p = small[i];
-        uint32_t *e = tbl + (size_t)i * TD_TBL;
+        uint32_t *e = tbl + i;          // field f at e[f * TD_SMALL_CAP]
         const uint32_t ra = td_powmod16(10u, (uint64_t)a, p);
         const uint32_t rb = td_powmod16(10u, (uint64_t)b, p);
         const uint32_t rc = td_powmod16(10u, (uint64_t)c, p);
@@ -162,18 +166,18 @@ void td_small_setup_kernel(const uint32_t *__restrict__ small,
             const uint32_t jr = ec >= 0 ? td_powmod16(10u, (uint64_t)ec, p) : 0u;
             jinv[l - 1] = jr ? td_powmod16(jr, p - 2u, p) : 0u;
         }
-        e[TT_P] = p;
-        e[TT_MAGIC] = (uint32_t)(0x100000000ull / p);
-        e[TT_UN0] = un0;
-        e[TT_B] = bstep;
+        e[TT_P * TD_SMALL_CAP] = p;
+        e[TT_MAGIC * TD_SMALL_CAP] = (uint32_t)(0x100000000ull / p);
+        e[TT_UN0 * TD_SMALL_CAP] = un0;
+        e[TT_B * TD_SMALL_CAP] = bstep;
         for (int lj = 0; lj < 4; lj++)
-            e[TT_CM + lj] = (ra1 * jinv[lj] % p + p - 1u) % p;
+            e[(TT_CM + lj) * TD_SMALL_CAP] = (ra1 * jinv[lj] % p + p - 1u) % p;
         for (int lk = 0; lk < 4; lk++)
             for (int lj = 0; lj < 4; lj++) {
-                e[TT_E + lk * 4 + lj] = kres[lk] * jinv[lj] % p;
+                e[(TT_E + lk * 4 + lj) * TD_SMALL_CAP] = kres[lk] * jinv[lj] % p;
                 if (kres[lk] != 0u && jinv[lj] != 0u) flags |= 1u << (1 + lk * 4 + lj);
             }
-        e[TT_FLAGS] = flags;
+        e[TT_FLAGS * TD_SMALL_CAP] = flags;
     }
 }
 
@@ -206,17 +210,20 @@ void td_rows_kernel(const uint32_t *__restrict__ tbl, const uint32_t *__restrict
         // LDS-relative bit of (k, j) is rowbit + j.
         const uint32_t rowbit = (k - 1u) * J - (w0 << 5) - 1u;
         for (uint32_t i = slot; i < count; i += nslots) {
-            const uint32_t *e = tbl + (size_t)i * TD_TBL;
-            const uint32_t p = e[TT_P], magic = e[TT_MAGIC], flags = e[TT_FLAGS];
+            const uint32_t *e = tbl + i;
+            const uint32_t p = e[TT_P * TD_SMALL_CAP], magic = e[TT_MAGIC * TD_SMALL_CAP];
+            const uint32_t flags = e[TT_FLAGS * TD_SMALL_CAP];
             if (flags & 1u) {
-                const uint32_t u = pf_submod(e[TT_UN0], td_mod(k * e[TT_B], p, magic), p);
+                const uint32_t u = pf_submod(e[TT_UN0 * TD_SMALL_CAP],
+                                             td_mod(k * e[TT_B * TD_SMALL_CAP], p, magic), p);
                 for (uint32_t j = u + 1u; j <= J; j += p) td_lds_mark(lds, rowbit + j);
             }
 #pragma unroll
             for (int lj = 1; lj <= DJ; lj++) {
                 if (!((flags >> (1 + (lk - 1) * 4 + (lj - 1))) & 1u)) continue;
-                const uint32_t ev = e[TT_E + (lk - 1) * 4 + (lj - 1)];
-                const uint32_t u = pf_submod(e[TT_CM + lj - 1], td_mod(rk * ev, p, magic), p);
+                const uint32_t ev = e[(TT_E + (lk - 1) * 4 + (lj - 1)) * TD_SMALL_CAP];
+                const uint32_t u = pf_submod(e[(TT_CM + lj - 1) * TD_SMALL_CAP],
+                                             td_mod(rk * ev, p, magic), p);
                 const uint32_t lim = td_pow10(lj);
                 for (uint32_t r = u + 1u; r < lim; r += p) {
                     if (r % 10u == 0u) continue;
@@ -239,34 +246,73 @@ __device__ __forceinline__ void td_gmark(uint32_t *bits, uint32_t bit)
                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Rare path of the lane kernel: emit every hit of row k.
-template <int DJ>
+// Rare path of the lane kernel: emit every hit of row k.  The lane kernel
+// only sees p >= TD_SMALL_LIMIT > 9999 >= max(J, 10^D_j - 1), so each
+// residue names at most one j.
+//
+// Separate groups (MERGED = false): residue l belongs to j digit group l+1;
+// its value v is rev(j) - 1 for the j it eliminates, and rev(j) must not end
+// in 0 (such a value belongs to no (l+1)-digit j).
+// Merged groups (MERGED = true, needs a-c >= DJ): one residue v = R - 1 with
+// R = rev(j) 10^(DJ - lj), i.e. rev(j) padded with zeros to DJ digits, which
+// names (lj, rev(j)) uniquely; rdec[R] holds that j (td_rdec_kernel).
+static_assert(TD_SMALL_LIMIT > 9999u, "lane kernel assumes one j per residue");
+
+template <int DJ, bool MERGED, int NR>
 __device__ __forceinline__ void td_hits(uint32_t k, uint32_t un, uint32_t thrn,
-                                        const uint32_t (&v)[DJ], const uint32_t (&thr)[DJ],
-                                        uint32_t p, uint32_t J, uint32_t *bits)
+                                        const uint32_t (&v)[NR], const uint32_t (&thr)[NR],
+                                        uint32_t J, const uint16_t *__restrict__ rdec,
+                                        uint32_t *bits)
 {
     const uint32_t rowbit = (k - 1u) * J - 1u;      // bit of (k, j) = rowbit + j
-    if (un < thrn)
-        for (uint32_t j = un + 1u; j <= J; j += p) td_gmark(bits, rowbit + j);
+    if (un < thrn) td_gmark(bits, rowbit + un + 1u);
 #pragma unroll
-    for (int l = 0; l < DJ; l++) {
+    for (int l = 0; l < NR; l++) {
         if (v[l] < thr[l]) {
-            const uint32_t lim = td_pow10(l + 1);
-            for (uint32_t r = v[l] + 1u; r < lim; r += p) {
-                if (r % 10u == 0u) continue;
-                const uint32_t j = td_reverse(r, l + 1);
-                if (j <= J) td_gmark(bits, rowbit + j);
+            const uint32_t r = v[l] + 1u;
+            uint32_t j;
+            if (MERGED) {
+                j = rdec[r];
+            } else {
+                j = (r % 10u == 0u) ? 0xFFFFFFFFu : td_reverse(r, l + 1);
             }
+            if (j <= J) td_gmark(bits, rowbit + j);
         }
     }
 }
 
-template <int DJ>
+// rdec tables for D = 1..4 at offsets 0, 10, 110, 1110: rdec_D[R] = the j
+// whose rev(j) padded with zeros to D digits is R (0 for R = 0).
+constexpr uint32_t TD_RDEC_SIZE = 11110;
+
+__device__ __forceinline__ uint32_t td_rdec_offset(int d)
+{
+    return d <= 1 ? 0u : d == 2 ? 10u : d == 3 ? 110u : 1110u;
+}
+
+__global__ void td_rdec_kernel(uint16_t *rdec)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= TD_RDEC_SIZE) return;
+    const int d = i < 10u ? 1 : i < 110u ? 2 : i < 1110u ? 3 : 4;
+    uint32_t r = i - td_rdec_offset(d);
+    uint32_t j = 0;
+    if (r != 0u) {
+        int lj = d;
+        while (r % 10u == 0u) { r /= 10u; lj--; }
+        j = td_reverse(r, lj);
+    }
+    rdec[i] = (uint16_t)j;
+}
+
+template <int DJ, bool MERGED>
 __global__ __launch_bounds__(TD_BLOCK)
 void td_large_kernel(const uint32_t *__restrict__ primes, uint32_t n,
                      long long a, long long b, long long c,
-                     uint32_t K, uint32_t J, uint32_t *__restrict__ bits)
+                     uint32_t K, uint32_t J, const uint16_t *__restrict__ rdec,
+                     uint32_t *__restrict__ bits)
 {
+    constexpr int NR = MERGED ? 1 : DJ;     // rev residues tracked per k
     const uint32_t i = blockIdx.x * TD_BLOCK + threadIdx.x;
     if (i >= n) return;
     const uint32_t p = primes[i];
@@ -288,15 +334,17 @@ void td_large_kernel(const uint32_t *__restrict__ primes, uint32_t n,
     uint32_t un = pf_submod(pf_mont_mul(ra1M, icS, m), 1u, p);                   // u at k = 0
     const uint32_t thrn = J;
 
-    // j digit groups: 10^{-(a-c-l)} (Montgomery) and Cm = (ra-1) 10^{-(a-c-l)} - 1.
-    uint32_t jinvM[DJ], cm[DJ];
-    bool jok[DJ];
+    // Rev residue l is (ra - 1 - rev(k) kres) / X_l - 1 with
+    //   separate: X_l = 10^(a-c-(l+1)), hit range [0, 10^(l+1) - 1);
+    //   merged:   X_0 = 10^(a-c-DJ),    hit range [0, 10^DJ - 1).
+    uint32_t jinvM[NR], cm[NR];
+    bool jok[NR];
     {
-        const long long e1 = a - c - 1;
+        const long long e1 = MERGED ? a - c - DJ : a - c - 1;
         uint32_t z = e1 >= 0 ? pf_mont_pow(tenInv, (uint64_t)e1, m) : 0u;
 #pragma unroll
-        for (int l = 0; l < DJ; l++) {
-            jok[l] = (a - c - (l + 1)) >= 0;
+        for (int l = 0; l < NR; l++) {
+            jok[l] = MERGED ? e1 >= 0 : (a - c - (l + 1)) >= 0;
             jinvM[l] = z;
             cm[l] = pf_submod(pf_mont_mul(ra1M, pf_mont_out(z, m), m), 1u, p);
             z = pf_mont_mul(z, ten, m);
@@ -314,11 +362,11 @@ void td_large_kernel(const uint32_t *__restrict__ primes, uint32_t n,
         // Steps of rev(k) inside a row of ten (units digit of k -> top digit
         // of rev(k)) and between rows (q -> q+1 with t9 trailing nines).
         const uint32_t s0w = td_pow10(lk - 1);
-        uint32_t thr[DJ], s0[DJ], w[DJ], tq0[DJ], tq1[DJ], tq2[DJ];
+        uint32_t thr[NR], s0[NR], w[NR], tq0[NR], tq1[NR], tq2[NR];
 #pragma unroll
-        for (int l = 0; l < DJ; l++) {
-            const uint32_t eM = pf_mont_mul(kresM, jinvM[l], m);      // E = kres/jres
-            thr[l] = (kok && jok[l]) ? td_pow10(l + 1) - 1u : 0u;
+        for (int l = 0; l < NR; l++) {
+            const uint32_t eM = pf_mont_mul(kresM, jinvM[l], m);      // E = kres / X_l
+            thr[l] = (kok && jok[l]) ? td_pow10(MERGED ? DJ : l + 1) - 1u : 0u;
             s0[l] = pf_mont_mul(eM, s0w, m);
             const uint32_t eS = pf_mont_out(eM, m);
             w[l] = lk == 1 ? cm[l] : pf_submod(cm[l], eS, p);         // rev(10^(lk-1)) = 1
@@ -336,40 +384,41 @@ void td_large_kernel(const uint32_t *__restrict__ primes, uint32_t n,
             const uint32_t kb = q * 10u;
             const uint32_t dlo = kb < klo ? klo - kb : 0u;
             const uint32_t dhi = min(9u, khi - kb);
-            uint32_t v[DJ];
+            uint32_t v[NR];
 #pragma unroll
-            for (int l = 0; l < DJ; l++) v[l] = w[l];
+            for (int l = 0; l < NR; l++) v[l] = w[l];
             if (dlo == 0u && dhi == 9u) {
 #pragma unroll
                 for (uint32_t d = 0; d < 10u; d++) {
                     un = pf_submod(un, bstep, p);
                     bool h = un < thrn;
 #pragma unroll
-                    for (int l = 0; l < DJ; l++) h |= v[l] < thr[l];
-                    if (__builtin_expect(h, 0)) td_hits<DJ>(kb + d, un, thrn, v, thr, p, J, bits);
+                    for (int l = 0; l < NR; l++) h |= v[l] < thr[l];
+                    if (__builtin_expect(h, 0))
+                        td_hits<DJ, MERGED, NR>(kb + d, un, thrn, v, thr, J, rdec, bits);
 #pragma unroll
-                    for (int l = 0; l < DJ; l++) v[l] = pf_submod(v[l], s0[l], p);
+                    for (int l = 0; l < NR; l++) v[l] = pf_submod(v[l], s0[l], p);
                 }
             } else {
                 for (uint32_t d = 0; d < dlo; d++) {
 #pragma unroll
-                    for (int l = 0; l < DJ; l++) v[l] = pf_submod(v[l], s0[l], p);
+                    for (int l = 0; l < NR; l++) v[l] = pf_submod(v[l], s0[l], p);
                 }
                 for (uint32_t d = dlo; d <= dhi; d++) {
                     un = pf_submod(un, bstep, p);
                     bool h = un < thrn;
 #pragma unroll
-                    for (int l = 0; l < DJ; l++) h |= v[l] < thr[l];
-                    if (h) td_hits<DJ>(kb + d, un, thrn, v, thr, p, J, bits);
+                    for (int l = 0; l < NR; l++) h |= v[l] < thr[l];
+                    if (h) td_hits<DJ, MERGED, NR>(kb + d, un, thrn, v, thr, J, rdec, bits);
 #pragma unroll
-                    for (int l = 0; l < DJ; l++) v[l] = pf_submod(v[l], s0[l], p);
+                    for (int l = 0; l < NR; l++) v[l] = pf_submod(v[l], s0[l], p);
                 }
             }
             if (q < qhi) {
                 uint32_t t9 = 0, qq = q;
                 while (qq % 10u == 9u) { t9++; qq /= 10u; }
 #pragma unroll
-                for (int l = 0; l < DJ; l++)
+                for (int l = 0; l < NR; l++)
                     w[l] = pf_submod(w[l], t9 == 0 ? tq0[l] : t9 == 1 ? tq1[l] : tq2[l], p);
             }
         }
@@ -382,11 +431,13 @@ void td_large_kernel(const uint32_t *__restrict__ primes, uint32_t n,
 // ---------------------------------------------------------------------------
 
 constexpr int TD_MAX_DEV = 64;
+constexpr uint32_t TD_RDEC_OFF[5] = {0u, 0u, 10u, 110u, 1110u};
 
 struct TdScratch {
     uint32_t *small = nullptr;   // [TD_SMALL_CAP]
     uint32_t *tbl = nullptr;     // [TD_SMALL_CAP * TD_TBL]
     uint32_t *ctrl = nullptr;    // [CT_WORDS]
+    uint16_t *rdec = nullptr;    // [TD_RDEC_SIZE]
 };
 
 TdScratch g_scratch[TD_MAX_DEV];
@@ -401,6 +452,9 @@ int td_scratch(TdScratch **out)
         if (hipMalloc(&s.tbl, (size_t)TD_SMALL_CAP * TD_TBL * sizeof(uint32_t)) != hipSuccess) return -1;
         if (hipMalloc(&s.ctrl, CT_WORDS * sizeof(uint32_t)) != hipSuccess) return -1;
         if (hipMemset(s.ctrl, 0, CT_WORDS * sizeof(uint32_t)) != hipSuccess) return -1;
+        if (hipMalloc(&s.rdec, TD_RDEC_SIZE * sizeof(uint16_t)) != hipSuccess) return -1;
+        td_rdec_kernel<<<(TD_RDEC_SIZE + 255u) / 256u, 256>>>(s.rdec);
+        if (hipDeviceSynchronize() != hipSuccess) return -1;
     }
     *out = &s;
     return 0;
@@ -415,9 +469,14 @@ void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long
     const size_t lds = ((size_t)TD_ROWS * J / 32u + 2u) * sizeof(uint32_t);
     td_rows_kernel<DJ><<<row_groups, TD_BLOCK, lds, st>>>(s.tbl, s.ctrl, K, J, d_bits);
     if (ev0) (void)hipEventRecord(ev0, st);
-    if (n)
-        td_large_kernel<DJ><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
-            d_primes, n, a, b, c, K, J, d_bits);
+    // One merged rev residue per k needs every j digit group's exponent
+    // a-c-lj >= 0; otherwise track the groups separately.
+    if (n && a - c >= DJ)
+        td_large_kernel<DJ, true><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
+            d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits);
+    else if (n)
+        td_large_kernel<DJ, false><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
+            d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits);
     if (ev1) (void)hipEventRecord(ev1, st);
 }
 

@@ -65,6 +65,7 @@ def lib() -> C.CDLL:
         "pf_memcpy_dtod": (C.c_int, [vp, vp, sz]),
         "pf_memset": (C.c_int, [vp, C.c_int, sz]),
         "pf_sync": (C.c_int, []),
+        "pf_device_copy": (C.c_int, [vp, vp, sz, vp]),
         "pf_stream_create": (vp, []),
         "pf_stream_destroy": (C.c_int, [vp]),
         "pf_stream_sync": (C.c_int, [vp]),
@@ -186,3 +187,134 @@ def device_info() -> dict:
     L.pf_device_arch(arch, 64)
     return {"name": (L.pf_device_name() or b"").decode(), "arch": arch.value.decode(),
             "cus": L.pf_device_cu_count(), "count": L.pf_device_count()}
+
+
+# ---------------------------------------------------------------- Miller-Rabin
+
+class MrCandidate(C.Structure):
+    """``mr_candidate_t`` (include/mr_gpu.h)."""
+    _fields_ = [("a", C.c_int64), ("b", C.c_int64), ("c", C.c_int64),
+                ("k", C.c_uint32), ("j", C.c_uint32), ("which", C.c_int32)]
+
+
+def _mr_sigs(L):
+    vp, i32, u32 = C.c_void_p, C.c_int, C.c_uint32
+    sig = {
+        "mr_ctx_create": (vp, [u32p, i32, i32, i32]),
+        "mr_ctx_create_limbs16": (vp, [C.POINTER(C.c_uint64), i32, i32, i32]),
+        "mr_ctx_create_sparse": (vp, [C.POINTER(MrCandidate), i32, i32]),
+        "mr_run": (i32, [vp, u32p, i32, C.POINTER(C.c_uint8), u32p]),
+        "mr_ctx_info": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                              C.POINTER(C.c_int32)]),
+        "mr_ctx_moduli": (i32, [vp, u32p]),
+        "mr_ctx_destroy": (None, [vp]),
+        "mr_debug_product": (i32, [vp, u32p, u32p, u32p]),
+        "mr_debug_montmul": (i32, [vp, u32p, u32p, u32p]),
+        "mr_debug_mulsmall": (i32, [vp, u32p, u32, u32p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def int_to_words(x: int, n_words: int) -> np.ndarray:
+    if x < 0 or x.bit_length() > 32 * n_words:
+        raise ValueError("integer does not fit")
+    return np.frombuffer(x.to_bytes(4 * n_words, "little"), dtype=np.uint32).copy()
+
+
+def words_to_int(w: np.ndarray) -> int:
+    return int.from_bytes(np.ascontiguousarray(w, dtype=np.uint32).tobytes(), "little")
+
+
+class MrContext:
+    """A batch of moduli resident on the GPU (include/mr_gpu.h)."""
+
+    def __init__(self, moduli=None, n_words: int | None = None, device: int = 0,
+                 candidates=None):
+        L = lib()
+        if not hasattr(L, "_mr_ready"):
+            _mr_sigs(L)
+            L._mr_ready = True
+        self.L = L
+        if candidates is not None:
+            arr = (MrCandidate * len(candidates))(*[MrCandidate(*c) for c in candidates])
+            self.ptr = L.mr_ctx_create_sparse(arr, len(candidates), device)
+        else:
+            moduli = [int(m) for m in moduli]
+            nw = n_words or max(1, max((m.bit_length() + 31) // 32 for m in moduli))
+            w = np.concatenate([int_to_words(m, nw) for m in moduli])
+            self.ptr = L.mr_ctx_create(w.ctypes.data_as(u32p), nw, len(moduli), device)
+        if not self.ptr:
+            raise RuntimeError("mr_ctx_create failed")
+        nw_, limbs, logm = C.c_int(), C.c_int(), C.c_int()
+        L.mr_ctx_info(self.ptr, C.byref(nw_), C.byref(limbs), C.byref(logm), None)
+        self.n_words, self.limbs, self.log_len = nw_.value, limbs.value, logm.value
+        self.n = len(candidates) if candidates is not None else len(moduli)
+        sv = (C.c_int32 * self.n)()
+        L.mr_ctx_info(self.ptr, None, None, None, sv)
+        self.s = list(sv)
+
+    @property
+    def R(self) -> int:
+        return 1 << (24 * self.limbs)
+
+    def moduli(self):
+        out = np.zeros(self.n * self.n_words, np.uint32)
+        assert self.L.mr_ctx_moduli(self.ptr, out.ctypes.data_as(u32p)) == 0
+        return [words_to_int(out[i * self.n_words:(i + 1) * self.n_words]) for i in range(self.n)]
+
+    def run(self, bases, residues: bool = True):
+        bases = np.asarray(bases, dtype=np.uint32)
+        alive = np.zeros(self.n, np.uint8)
+        res = np.zeros(self.n * len(bases) * self.n_words, np.uint32) if residues else None
+        rc = self.L.mr_run(self.ptr, bases.ctypes.data_as(u32p), len(bases),
+                           alive.ctypes.data_as(C.POINTER(C.c_uint8)),
+                           res.ctypes.data_as(u32p) if residues else None)
+        if rc != 0:
+            raise RuntimeError(f"mr_run failed ({rc})")
+        if not residues:
+            return alive.astype(bool).tolist(), None
+        nw, nb = self.n_words, len(bases)
+        rr = [[words_to_int(res[(i * nb + b) * nw:(i * nb + b + 1) * nw]) for b in range(nb)]
+              for i in range(self.n)]
+        return alive.astype(bool).tolist(), rr
+
+    def _words(self, xs):
+        return np.concatenate([int_to_words(int(x), self.n_words) for x in xs])
+
+    def debug_product(self, a, b):
+        out = np.zeros(self.n * 2 * self.n_words, np.uint32)
+        A, B = self._words(a), self._words(b)
+        assert self.L.mr_debug_product(self.ptr, A.ctypes.data_as(u32p), B.ctypes.data_as(u32p),
+                                       out.ctypes.data_as(u32p)) == 0
+        w = 2 * self.n_words
+        return [words_to_int(out[i * w:(i + 1) * w]) for i in range(self.n)]
+
+    def debug_montmul(self, a, b):
+        out = np.zeros(self.n * self.n_words, np.uint32)
+        A, B = self._words(a), self._words(b)
+        assert self.L.mr_debug_montmul(self.ptr, A.ctypes.data_as(u32p), B.ctypes.data_as(u32p),
+                                       out.ctypes.data_as(u32p)) == 0
+        w = self.n_words
+        return [words_to_int(out[i * w:(i + 1) * w]) for i in range(self.n)]
+
+    def debug_mulsmall(self, a, s: int):
+        out = np.zeros(self.n * self.n_words, np.uint32)
+        A = self._words(a)
+        assert self.L.mr_debug_mulsmall(self.ptr, A.ctypes.data_as(u32p), s,
+                                        out.ctypes.data_as(u32p)) == 0
+        w = self.n_words
+        return [words_to_int(out[i * w:(i + 1) * w]) for i in range(self.n)]
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.L.mr_ctx_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -146,3 +146,83 @@ def survivors(bits: np.ndarray, K: int, J: int):
 
 def popcount(bits: np.ndarray) -> int:
     return int(np.unpackbits(bits.view(np.uint8)).sum())
+
+
+# ------------------------------------------------------------- GMP oracle
+_mr_oracle_ready = False
+
+
+def _mr_oracle():
+    global _mr_oracle_ready
+    L = oracle()
+    if not _mr_oracle_ready:
+        sz = C.c_size_t
+        L.mr_oracle_build.argtypes = [ll, ll, ll, C.c_ulong, C.c_ulong, C.c_int, u32p, sz]
+        L.mr_oracle_build.restype = C.c_long
+        L.mr_oracle_sprp.argtypes = [u32p, sz, C.c_ulong, u32p, C.POINTER(C.c_int)]
+        L.mr_oracle_sprp.restype = C.c_int
+        L.mr_oracle_probab_prime.argtypes = [u32p, sz, C.c_int]
+        L.mr_oracle_probab_prime.restype = C.c_int
+        L.mr_oracle_powm.argtypes = [u32p, sz, u32p, u32p, sz, u32p]
+        L.mr_oracle_powm.restype = C.c_int
+        L.mr_oracle_gmp_version.restype = C.c_char_p
+        _mr_oracle_ready = True
+    return L
+
+
+def int_words(x: int, n: int) -> np.ndarray:
+    return np.frombuffer(x.to_bytes(4 * n, "little"), dtype=np.uint32).copy()
+
+
+def words_int(w) -> int:
+    return int.from_bytes(np.ascontiguousarray(w, dtype=np.uint32).tobytes(), "little")
+
+
+def gmp_build(a, b, c, k, j, mode=0) -> int:
+    """N (mode 0), decimal reversal of N (1) or the reference rev formula (2)."""
+    cap = (int(a * 3.33) + 64) // 32 + 4
+    out = np.zeros(cap, np.uint32)
+    n = _mr_oracle().mr_oracle_build(a, b, c, k, j, mode, out.ctypes.data_as(u32p), cap)
+    if n < 0:
+        raise ValueError("mr_oracle_build failed")
+    return words_int(out)
+
+
+def gmp_sprp(N: int, base: int):
+    """(verdict, base^d mod N, s) from GMP mpz_powm."""
+    nw = (N.bit_length() + 31) // 32
+    w = int_words(N, nw)
+    res = np.zeros(nw, np.uint32)
+    s = C.c_int()
+    v = _mr_oracle().mr_oracle_sprp(w.ctypes.data_as(u32p), nw, base, res.ctypes.data_as(u32p),
+                                    C.byref(s))
+    return v, words_int(res), s.value
+
+
+def gmp_probab_prime(N: int, reps: int = 5) -> int:
+    nw = (N.bit_length() + 31) // 32
+    w = int_words(N, nw)
+    return _mr_oracle().mr_oracle_probab_prime(w.ctypes.data_as(u32p), nw, reps)
+
+
+def gmp_version() -> str:
+    return _mr_oracle().mr_oracle_gmp_version().decode()
+
+
+def sprp_python(N: int, base: int):
+    """Textbook strong-probable-prime round with Python integers."""
+    d, s = N - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    r = pow(base, d, N)
+    if r == 1 or r == N - 1:
+        return True, r, s
+    x = r
+    for _ in range(s - 1):
+        x = x * x % N
+        if x == N - 1:
+            return True, r, s
+        if x == 1:
+            break
+    return False, r, s

@@ -70,7 +70,8 @@ def test_edge_exponents_vs_oracle(gpu):
     """a-b-l and a-c-l at and below zero; 2 and 5 in the list."""
     primes = numpy_primes(100000)
     for (a, b, c) in [(104, 101, 100), (103, 100, 1), (7, 4, 2), (6, 3, 1), (4, 3, 1),
-                      (1005, 1001, 1000), (1004, 1000, 996), (10, 0, 0), (5000, 4999, 4998)]:
+                      (1005, 1001, 1000), (1004, 1000, 996), (10, 0, 0), (5000, 4999, 4998),
+                      (1003, 1001, 1000), (1002, 1001, 1000), (50004, 50001, 50000)]:
         say("edge", a, b, c)
         for (K, J) in [(100, 100), (1500, 1500)]:
             rc, got = gpu.trial_div_gpu_kj(primes, a, b, c, K, J)
