@@ -70,6 +70,30 @@ def cpu_baseline(primes: np.ndarray, triple, budget_s: float) -> dict:
                        f"oracle/td_oracle.c restating trial_div_cpu.c")}
 
 
+def mr_leg(pf, n_bits: int) -> dict:
+    """Batched Miller-Rabin at 50 000 digits (workload 3: 128 seeded
+    candidates and their decimal reversals, built on the device): time the
+    first n_bits squaring steps of a base-2 round over the whole batch and
+    scale to the full exponent length.  Reported beside the headline."""
+    cfg = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())["3"]
+    cands = [tuple(x) + (w,) for x in cfg["candidates"] for w in (0, 1)]
+    t0 = time.perf_counter()
+    ctx = pf.MrContext(candidates=cands)
+    setup_s = time.perf_counter() - t0
+    ctx.time_squarings(2, 64)                         # warm-up
+    ms = ctx.time_squarings(2, n_bits)
+    bits = 166096                                     # squarings per 50 000-digit test
+    test_s = ms / 1e3 / n_bits * bits
+    res = {"workload": "256 bigints (128 N + 128 rev N) at 50 000 digits, base 2",
+           "limbs": ctx.limbs, "transform_len": 1 << ctx.log_len,
+           "us_per_squaring_batch": ms * 1e3 / n_bits,
+           "tests_per_hour_extrapolated": 3600.0 * len(cands) / test_s,
+           "sample": f"{n_bits} of {bits} squaring steps timed (HIP events), scaled",
+           "setup_s": setup_s}
+    ctx.close()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +101,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mr-bits", type=int, default=2048,
+                    help="squaring steps timed per Miller-Rabin sample (0 = skip the MR leg)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain copy kernel of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -169,6 +195,8 @@ def main() -> int:
         "survivors_first_triple": survivors0,
         "device": pf.device_info(),
     }
+    if args.mr_bits:
+        out["mr"] = mr_leg(pf, args.mr_bits)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(primes, triples[0], args.cpu_seconds)
     print(json.dumps(out), flush=True)

@@ -33,8 +33,10 @@ extern "C" {
 typedef struct mr_ctx mr_ctx_t;
 
 /* Build a context from n_batch moduli, each n_words little-endian uint32
- * words, laid out [n_batch][n_words].  Every modulus must be odd and > 3;
- * bit length <= 384 000.  device = HIP device index.  NULL on error.
+ * words, laid out [n_batch][n_words].  Bit length <= 196 000 in this build
+ * (transforms up to 2^14).  Even moduli and 0, 1 are reported composite and
+ * 2, 3 prime without running rounds.  device = HIP device index.  NULL on
+ * error.
  * Replaces BatchModCtx(N_all, n_limbs, n_batch, device)
  * (batch_mod_ctx.cuh:132-133). */
 mr_ctx_t *mr_ctx_create(const uint32_t *N_words, int n_words, int n_batch, int device);
@@ -75,6 +77,12 @@ int mr_ctx_info(const mr_ctx_t *ctx, int *n_words, int *limbs, int *log_len, int
 int mr_ctx_moduli(const mr_ctx_t *ctx, uint32_t *out);
 
 void mr_ctx_destroy(mr_ctx_t *ctx);
+
+/* Benchmark helper (no counterpart in the reference, whose runner prints
+ * per-phase timings itself, miller_rabin_runner.cu:205-275): starts a round
+ * with `base` on every candidate and times its first n_bits squaring steps
+ * on the context's stream; *ms = elapsed milliseconds. */
+int mr_time_squarings(mr_ctx_t *ctx, uint32_t base, int n_bits, float *ms);
 
 /* Arithmetic self-check entry points (used by the parity tests):
  * out = a * b (full product), a * b * R^-1 mod N, or a * s mod N for a

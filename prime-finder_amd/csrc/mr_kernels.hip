@@ -46,6 +46,7 @@ struct Dev {                    // per-batch device state (kernel argument)
     int32_t *sval;              // [n] s with N - 1 = 2^s d
     int32_t *bitpos;            // [n] next exponent bit (bit index of N), < s when done
     uint8_t *alive;             // [n]
+    uint8_t *pre;               // [n] 0 = test, 1 = prime (2, 3), 2 = composite (even, < 2)
     uint32_t *err;              // [1] error latch
 };
 
@@ -278,6 +279,109 @@ __device__ void montmul(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, cons
     }
 }
 
+// x <- x * s (plain integer product), 1 <= s < 2^24.
+template <int C>
+__device__ void mul_small_vec(uint32_t (&x)[C], uint32_t s, Scratch &sc)
+{
+    const int t = threadIdx.x;
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+        acc += (uint64_t)x[i] * s;
+        x[i] = (uint32_t)acc & LIMB_MASK;
+        acc >>= LIMB_BITS;
+    }
+    sc.carry[t] = acc;                  // < 2^24 + 2
+    __syncthreads();
+    uint64_t cin = t ? sc.carry[t - 1] : 0ull;
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+        cin += x[i];
+        x[i] = (uint32_t)cin & LIMB_MASK;
+        cin >>= LIMB_BITS;
+    }
+    bool all1 = true;
+#pragma unroll
+    for (int i = 0; i < C; i++) all1 &= (x[i] == LIMB_MASK);
+    uint32_t cc = block_carry_in(cin != 0ull, all1, sc);
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+        const uint32_t v = x[i] + cc;
+        x[i] = v & LIMB_MASK;
+        cc = v >> LIMB_BITS;
+    }
+}
+
+// x <- x * 2^e through the LDS buffer X (M words).
+template <int C>
+__device__ void shl_vec(uint32_t (&x)[C], long e, uint32_t *X, int M)
+{
+    const int p0 = threadIdx.x * C;
+#pragma unroll
+    for (int i = 0; i < C; i++) X[p0 + i] = x[i];
+    __syncthreads();
+    const long ls = e / 24;
+    const int bs = (int)(e % 24);
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+        const long k = (long)(p0 + i) - ls;
+        const uint32_t hi = (k >= 0 && k < M) ? X[k] : 0u;
+        const uint32_t lo = (k - 1 >= 0 && k - 1 < M) ? X[k - 1] : 0u;
+        x[i] = bs ? (((hi << bs) | (lo >> (24 - bs))) & LIMB_MASK) : hi;
+    }
+    __syncthreads();
+}
+
+// x <- 5^e.
+template <int LOGM>
+__device__ void pow5(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, uint32_t (&x)[G<LOGM>::C],
+                     long e)
+{
+    constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
+#pragma unroll
+    for (int i = 0; i < C; i++) x[i] = 0u;
+    if (threadIdx.x == 0) x[0] = e ? 5u : 1u;
+    if (e <= 1) return;
+    int top = 62 - __clzll((long long)e);      // bit below the leading one
+    for (int b = top; b >= 0; b--) {
+        tproduct<LOGM>(sm, T, K, x, 0, M / 2, nullptr, x, M);
+        if ((e >> b) & 1) mul_small_vec<C>(x, 5u, sm.sc);
+    }
+}
+
+// N = sum of coef * 10^exp over the candidate's terms; one workgroup each.
+template <int LOGM>
+__global__ __launch_bounds__(THREADS) void build_kernel(Dev D, Twiddles T, Consts K,
+                                                        const int64_t *terms, int tmax)
+{
+    constexpr int C = G<LOGM>::C;
+    __shared__ Smem<LOGM> sm;
+    const int c = blockIdx.x;
+    uint32_t P[C], Q[C], x[C];
+#pragma unroll
+    for (int i = 0; i < C; i++) P[i] = Q[i] = 0u;
+    const int64_t *tc = terms + (size_t)c * tmax * 2;
+    for (int t = 0; t < tmax; t++) {
+        const int64_t coef = tc[2 * t], ex = tc[2 * t + 1];
+        if (coef == 0) continue;
+        pow5<LOGM>(sm, T, K, x, (long)ex);
+        const uint32_t mag = (uint32_t)(coef < 0 ? -coef : coef);
+        if (mag != 1u) mul_small_vec<C>(x, mag, sm.sc);
+        shl_vec<C>(x, (long)ex, sm.X, G<LOGM>::M);
+        if (coef > 0) add_vec<C>(P, x, 0u, sm.sc);
+        else add_vec<C>(Q, x, 0u, sm.sc);
+    }
+    sub_vec<C>(x, P, Q, sm.sc);
+    store_vec<C>(x, D.N + (size_t)c * D.stride, D.stride, 0);
+}
+
+__global__ void init_alive_kernel(Dev D)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < D.n) D.alive[c] = D.pre[c] == 2 ? 0 : 1;
+}
+
 // ------------------------------------------------------------------ kernels
 
 __global__ void twiddle_kernel(uint32_t *out, int logm, uint32_t w0, uint32_t w1)
@@ -366,14 +470,18 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
         if (low < (1 << 30)) atomicMin(&sm.bcast[1], (uint32_t)low);
         __syncthreads();
         if (threadIdx.x == 0) {
-            D.nbits[c] = (int32_t)sm.bcast[0];
+            const int nb = (int)sm.bcast[0];
+            D.nbits[c] = nb;
             D.sval[c] = (int32_t)sm.bcast[1];
-            if (!(Nl[0] & 1u) || sm.bcast[0] < 3 || (int)sm.bcast[0] > 24 * L - 1)
-                atomicOr(D.err, 1u);
+            // 2 and 3 are prime, 0, 1 and even numbers are not: no rounds run
+            const uint32_t n0 = Nl[0];
+            D.pre[c] = (nb <= 2) ? ((n0 == 2u || n0 == 3u) ? 1 : 2) : ((n0 & 1u) ? 0 : 2);
+            if (nb > 24 * L - 1) atomicOr(D.err, 1u);
         }
         __syncthreads();
     }
     const int nbits = (int)sm.bcast[0];
+    if (D.pre[c] != 0) return;
 
     uint32_t *Nf = D.Nf + (size_t)c * 2 * M;
     uint32_t *Npf = D.Npf + (size_t)c * 2 * M;
@@ -433,7 +541,7 @@ __global__ __launch_bounds__(THREADS) void round_start_kernel(Dev D, uint32_t ba
     constexpr int C = G<LOGM>::C;
     __shared__ Scratch sc;
     const int c = blockIdx.x;
-    if (!D.alive[c]) return;
+    if (!D.alive[c] || D.pre[c]) return;
     uint32_t r[C], nv[C];
     load_vec<C>(nv, D.N + (size_t)c * D.stride, D.L, 0);
     load_vec<C>(r, D.one + (size_t)c * D.stride, D.L, 0);
@@ -451,7 +559,7 @@ __global__ __launch_bounds__(THREADS) void exp_kernel(Dev D, Twiddles T, Consts 
     constexpr int C = G<LOGM>::C;
     __shared__ Smem<LOGM> sm;
     const int c = blockIdx.x;
-    if (!D.alive[c]) return;
+    if (!D.alive[c] || D.pre[c]) return;
     const int s = D.sval[c];
     int bp = D.bitpos[c];
     if (bp < s) return;
@@ -487,7 +595,7 @@ __global__ __launch_bounds__(THREADS) void round_end_kernel(Dev D, Twiddles T, C
     constexpr int C = G<LOGM>::C;
     __shared__ Smem<LOGM> sm;
     const int c = blockIdx.x;
-    if (!D.alive[c]) return;
+    if (!D.alive[c] || D.pre[c]) return;
     const int L = D.L;
     uint32_t r[C], a[C], b[C];
     load_vec<C>(r, D.res + (size_t)c * D.stride, L, L);
@@ -660,16 +768,15 @@ int ctx_alloc(mr_ctx *x, void **p, size_t bytes)
     return 0;
 }
 
-// Common construction once N words are resident at dN_words ([n][n_words]).
-int ctx_init(mr_ctx *x, const uint32_t *h_words)
+// Geometry and device arrays for numbers of at most `bits` bits.
+int ctx_alloc_all(mr_ctx *x, long bits)
 {
-    const int bits = x->n_words * 32;
-    int L = (bits + 23) / 24;
+    int L = (int)((bits + 23) / 24);
     if (L < 4) L = 4;
     int logm = MIN_LOGM;
     while ((1 << logm) < 2 * L + 2) logm++;
     if (logm > MAX_LOGM || L > MAX_LIMBS) {
-        fprintf(stderr, "%s: modulus of %d bits is beyond this build's transform sizes\n", kWho, bits);
+        fprintf(stderr, "%s: numbers of %ld bits are beyond this build's transform sizes\n", kWho, bits);
         return -1;
     }
     x->L = L;
@@ -696,18 +803,22 @@ int ctx_init(mr_ctx *x, const uint32_t *h_words)
     MR_ALLOC(sval, int32_t, n);
     MR_ALLOC(bitpos, int32_t, n);
     MR_ALLOC(alive, uint8_t, n);
+    MR_ALLOC(pre, uint8_t, n);
     MR_ALLOC(err, uint32_t, 1);
 #undef MR_ALLOC
-    uint32_t *dw = nullptr;
-    if (ctx_alloc(x, (void **)&dw, sizeof(uint32_t) * n * x->n_words) != 0) return -2;
-    MR_CHECK(hipMemcpyAsync(dw, h_words, sizeof(uint32_t) * n * x->n_words, hipMemcpyHostToDevice, x->st));
-    words_to_limbs<<<dim3(8, x->n), 256, 0, x->st>>>(dw, x->n_words, d.N, x->stride, L);
     uint32_t *tw = nullptr;
     if (get_twiddles(x->device, logm, x->st, &tw) != 0) return hip_fail(hipErrorUnknown, "twiddles");
     x->T.base = tw;
     x->T.half = M / 2;
     x->K = make_consts(logm);
-    int rc = dispatch_logm(logm, [&](auto lg) {
+    return 0;
+}
+
+// Per-candidate constants once the moduli are resident in d.N.
+int ctx_finish(mr_ctx *x)
+{
+    const size_t n = (size_t)x->n;
+    int rc = dispatch_logm(x->logm, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
         setup_kernel<LG><<<x->n, THREADS, 0, x->st>>>(x->d, x->T, x->K);
         return 0;
@@ -715,15 +826,27 @@ int ctx_init(mr_ctx *x, const uint32_t *h_words)
     if (rc) return rc;
     MR_CHECK(hipGetLastError());
     uint32_t err = 0;
-    MR_CHECK(hipMemcpyAsync(&err, d.err, 4, hipMemcpyDeviceToHost, x->st));
+    MR_CHECK(hipMemcpyAsync(&err, x->d.err, 4, hipMemcpyDeviceToHost, x->st));
     x->sval.resize(n);
-    MR_CHECK(hipMemcpyAsync(x->sval.data(), d.sval, 4 * n, hipMemcpyDeviceToHost, x->st));
+    MR_CHECK(hipMemcpyAsync(x->sval.data(), x->d.sval, 4 * n, hipMemcpyDeviceToHost, x->st));
     MR_CHECK(hipStreamSynchronize(x->st));
     if (err) {
-        fprintf(stderr, "%s: every modulus must be odd, > 3 and fit the context\n", kWho);
+        fprintf(stderr, "%s: a modulus does not fit the context\n", kWho);
         return -1;
     }
     return 0;
+}
+
+int ctx_init(mr_ctx *x, const uint32_t *h_words)
+{
+    int rc = ctx_alloc_all(x, (long)x->n_words * 32);
+    if (rc) return rc;
+    uint32_t *dw = nullptr;
+    const size_t n = (size_t)x->n;
+    if (ctx_alloc(x, (void **)&dw, sizeof(uint32_t) * n * x->n_words) != 0) return -2;
+    MR_CHECK(hipMemcpyAsync(dw, h_words, sizeof(uint32_t) * n * x->n_words, hipMemcpyHostToDevice, x->st));
+    words_to_limbs<<<dim3(8, x->n), 256, 0, x->st>>>(dw, x->n_words, x->d.N, x->stride, x->L);
+    return ctx_finish(x);
 }
 
 mr_ctx *ctx_new(int n_words, int n_batch, int device)
@@ -812,7 +935,7 @@ extern "C" int mr_run(mr_ctx_t *x, const uint32_t *bases, int n_bases, uint8_t *
         if (bases[i] < 2u || bases[i] >= (1u << 24)) return -1;
     MR_CHECK(hipSetDevice(x->device));
     const int n = x->n;
-    MR_CHECK(hipMemsetAsync(x->d.alive, 1, n, x->st));
+    init_alive_kernel<<<(n + 255) / 256, 256, 0, x->st>>>(x->d);
     uint32_t *dres = nullptr;
     uint32_t *dlimbs = nullptr;
     const size_t rbytes = sizeof(uint32_t) * (size_t)n * n_bases * x->n_words;
@@ -924,9 +1047,160 @@ extern "C" int mr_debug_mulsmall(mr_ctx_t *x, const uint32_t *a, uint32_t s, uin
     return debug_op(x, 2, a, nullptr, s, out);
 }
 
+namespace {
+
+int ndig_u(uint64_t v)
+{
+    int n = 1;
+    while (v >= 10u) { v /= 10u; n++; }
+    return n;
+}
+
+uint64_t rev_u(uint64_t v, int len)
+{
+    uint64_t r = 0;
+    for (int i = 0; i < len; i++) { r = r * 10u + v % 10u; v /= 10u; }
+    return r;
+}
+
+// value = sum coef * 10^exp (exponents distinct), and the decimal length
+// bound of the value.  Returns false for an invalid candidate.
+bool candidate_terms(const mr_candidate_t &cd, std::vector<std::pair<int64_t, int64_t>> &terms,
+                     int64_t *digits)
+{
+    const int64_t a = cd.a, b = cd.b, c = cd.c;
+    const uint64_t k = cd.k, j = cd.j;
+    if (a < 1 || b < 0 || c < 0 || k < 1 || j < 1 || k > 1000000u || j > 1000000u) return false;
+    std::vector<std::pair<int64_t, int64_t>> raw;
+    if (cd.which == 2) {
+        const int lk = ndig_u(k), lj = ndig_u(j);
+        const int64_t eb = a - b - lk, ec = a - c - lj;
+        if (eb < 0 || ec < 0 || a >= 2000000) return false;
+        raw = {{1, a}, {-(int64_t)rev_u(k, lk), eb}, {-(int64_t)rev_u(j, lj), ec}, {-1, 0}};
+        *digits = a;
+    } else if (cd.which == 0 || cd.which == 1) {
+        // S = k 10^b + j 10^c as sparse decimal digits; N = (10^a - 1) - S
+        // has digits 9 - s_i with no borrow, when S < 10^a - 1.
+        std::vector<std::pair<int64_t, int>> sd;   // (position, digit), s_i != 0
+        const int64_t lo = b < c ? b : c, hi = b < c ? c : b;
+        const uint64_t clo = b < c ? k : j, chi = b < c ? j : k;
+        if (hi - lo <= 12) {
+            uint64_t x = chi;
+            for (int64_t i = 0; i < hi - lo; i++) x *= 10u;
+            x += clo;
+            for (int64_t p = lo; x; p++, x /= 10u) if (x % 10u) sd.push_back({p, (int)(x % 10u)});
+        } else {
+            uint64_t x = clo;
+            for (int64_t p = lo; x; p++, x /= 10u) if (x % 10u) sd.push_back({p, (int)(x % 10u)});
+            x = chi;
+            for (int64_t p = hi; x; p++, x /= 10u) if (x % 10u) sd.push_back({p, (int)(x % 10u)});
+        }
+        for (auto &q : sd) if (q.first >= a) return false;          // S >= 10^a
+        // decimal length D of N: drop leading digits 9 - 9 = 0
+        int64_t D = a;
+        for (;;) {
+            if (D == 0) return false;
+            bool nine = false;
+            for (auto &q : sd) if (q.first == D - 1 && q.second == 9) nine = true;
+            if (!nine) break;
+            D--;
+        }
+        if (a >= 2000000) return false;
+        if (cd.which == 0) {
+            raw = {{1, a}, {-1, 0}};
+            for (auto &q : sd) raw.push_back({-(int64_t)q.second, q.first});
+            *digits = a;
+        } else {
+            raw = {{1, D}, {-1, 0}};
+            for (auto &q : sd)
+                if (q.first < D) raw.push_back({-(int64_t)q.second, D - 1 - q.first});
+            *digits = D;
+        }
+    } else {
+        return false;
+    }
+    terms.clear();
+    for (auto &r : raw) {
+        bool merged = false;
+        for (auto &t : terms)
+            if (t.second == r.second) { t.first += r.first; merged = true; }
+        if (!merged) terms.push_back(r);
+    }
+    return true;
+}
+
+} // namespace
+
 extern "C" mr_ctx_t *mr_ctx_create_sparse(const mr_candidate_t *cands, int n_batch, int device)
 {
-    (void)cands; (void)n_batch; (void)device;
-    fprintf(stderr, "%s: mr_ctx_create_sparse: device candidate builder not built yet\n", kWho);
-    return nullptr;
+    if (!cands || n_batch < 1) return nullptr;
+    const int TMAX = 24;
+    std::vector<int64_t> terms((size_t)n_batch * TMAX * 2, 0);
+    int64_t max_digits = 1;
+    for (int i = 0; i < n_batch; i++) {
+        std::vector<std::pair<int64_t, int64_t>> t;
+        int64_t dg = 0;
+        if (!candidate_terms(cands[i], t, &dg) || (int)t.size() > TMAX) {
+            fprintf(stderr, "%s: invalid candidate %d (a=%lld b=%lld c=%lld k=%u j=%u which=%d)\n",
+                    kWho, i, (long long)cands[i].a, (long long)cands[i].b, (long long)cands[i].c,
+                    cands[i].k, cands[i].j, cands[i].which);
+            return nullptr;
+        }
+        for (size_t q = 0; q < t.size(); q++) {
+            terms[((size_t)i * TMAX + q) * 2] = t[q].first;
+            terms[((size_t)i * TMAX + q) * 2 + 1] = t[q].second;
+        }
+        if (dg > max_digits) max_digits = dg;
+    }
+    // bits of 10^D: floor(D log2 10) + 1, bounded from above exactly enough
+    const long bits = (long)((double)max_digits * 3.3219280948873623) + 2;
+    mr_ctx *x = ctx_new((int)((bits + 31) / 32), n_batch, device);
+    if (!x) return nullptr;
+    int rc = ctx_alloc_all(x, (long)x->n_words * 32);
+    int64_t *dterms = nullptr;
+    if (rc == 0 && ctx_alloc(x, (void **)&dterms, sizeof(int64_t) * terms.size()) != 0) rc = -2;
+    if (rc == 0 && hipMemcpyAsync(dterms, terms.data(), sizeof(int64_t) * terms.size(),
+                                  hipMemcpyHostToDevice, x->st) != hipSuccess) rc = -2;
+    if (rc == 0)
+        rc = dispatch_logm(x->logm, [&](auto lg) {
+            constexpr int LG = decltype(lg)::value;
+            build_kernel<LG><<<x->n, THREADS, 0, x->st>>>(x->d, x->T, x->K, dterms, TMAX);
+            return 0;
+        });
+    if (rc == 0) rc = ctx_finish(x);
+    if (rc != 0) {
+        mr_ctx_destroy(x);
+        return nullptr;
+    }
+    return x;
+}
+
+extern "C" int mr_time_squarings(mr_ctx_t *x, uint32_t base, int n_bits, float *ms)
+{
+    if (!x || !ms || n_bits < 1 || base < 2u || base >= (1u << 24)) return -1;
+    MR_CHECK(hipSetDevice(x->device));
+    hipEvent_t e0, e1;
+    MR_CHECK(hipEventCreate(&e0));
+    MR_CHECK(hipEventCreate(&e1));
+    const int n = x->n;
+    init_alive_kernel<<<(n + 255) / 256, 256, 0, x->st>>>(x->d);
+    int rc = dispatch_logm(x->logm, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        round_start_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, base);
+        (void)hipEventRecord(e0, x->st);
+        const int CHUNK = 256;
+        for (int done = 0; done < n_bits; done += CHUNK) {
+            const int steps = n_bits - done < CHUNK ? n_bits - done : CHUNK;
+            exp_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, x->T, x->K, base, steps);
+        }
+        (void)hipEventRecord(e1, x->st);
+        return 0;
+    });
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    return e == hipSuccess ? 0 : hip_fail(e, "mr_time_squarings");
 }

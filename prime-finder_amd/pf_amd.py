@@ -211,6 +211,7 @@ def _mr_sigs(L):
         "mr_debug_product": (i32, [vp, u32p, u32p, u32p]),
         "mr_debug_montmul": (i32, [vp, u32p, u32p, u32p]),
         "mr_debug_mulsmall": (i32, [vp, u32p, u32, u32p]),
+        "mr_time_squarings": (i32, [vp, u32, i32, C.POINTER(C.c_float)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -307,6 +308,13 @@ class MrContext:
                                         out.ctypes.data_as(u32p)) == 0
         w = self.n_words
         return [words_to_int(out[i * w:(i + 1) * w]) for i in range(self.n)]
+
+    def time_squarings(self, base: int, n_bits: int) -> float:
+        """Milliseconds for the first n_bits squaring steps of a round."""
+        ms = C.c_float()
+        if self.L.mr_time_squarings(self.ptr, base, n_bits, C.byref(ms)) != 0:
+            raise RuntimeError("mr_time_squarings failed")
+        return ms.value
 
     def close(self):
         if getattr(self, "ptr", None):

@@ -132,27 +132,89 @@ def test_residues_vs_gmp_medium(gpu):
     ctx.close()
 
 
-def test_multiplicative_consistency_50k_digits(gpu):
-    """Size-independent checks at the 50 000-digit working size: residues
-    are multiplicative in the base, (ab)^d = a^d b^d mod N, and
+def test_multiplicative_consistency_20k_digits(gpu):
+    """Size-independent checks on candidate-form numbers of 20 000 digits:
+    residues are multiplicative in the base, (ab)^d = a^d b^d mod N, and
     4^d = (2^d)^2 mod N."""
-    rng = random.Random(50000)
+    rng = random.Random(20000)
     mods = []
     for _ in range(2):
-        b = rng.randint(1000, 49000)
+        b = rng.randint(1000, 19000)
         c = rng.randint(100, b - 1)
-        n = 10 ** 50000 - rng.randint(1, 4096) * 10 ** b - rng.randint(1, 4096) * 10 ** c - 1
+        n = 10 ** 20000 - rng.randint(1, 4096) * 10 ** b - rng.randint(1, 4096) * 10 ** c - 1
         mods += [n, int(str(n)[::-1])]
     ctx = gpu.MrContext(mods)
-    say("50k-digit consistency: L", ctx.limbs, "logM", ctx.log_len)
-    bases = [2, 3, 6, 4]
-    # run all four bases on every candidate regardless of earlier failures
     rows = []
-    for base in bases:
+    for base in (2, 3, 6, 4):      # every base on every candidate
         _, r = ctx.run([base])
         rows.append([x[0] for x in r])
+        say("20k-digit consistency: base", base, "done")
     for i, n in enumerate(mods):
         r2, r3, r6, r4 = rows[0][i], rows[1][i], rows[2][i], rows[3][i]
         assert r6 == r2 * r3 % n
         assert r4 == r2 * r2 % n
+    ctx.close()
+
+
+def test_50k_digit_residues_vs_gmp_golden(gpu):
+    """Workload-3 candidates at 50 000 digits (N and its reversal, built on
+    the device): base-2 residues equal GMP's (SHA-256 digests committed under
+    tests/golden, made by tests/golden/make_mr_golden.py)."""
+    import hashlib
+    import json
+    from _support import GOLDEN
+    gold = json.loads((GOLDEN / "mr_50k_residues.json").read_text())["rows"]
+    gold = [g for g in gold if g["base"] == 2]
+    cands = [(g["a"], g["b"], g["c"], g["k"], g["j"], g["which"]) for g in gold]
+    ctx = gpu.MrContext(candidates=cands)
+    say("50k-digit golden: L", ctx.limbs, "logM", ctx.log_len, "- one base-2 round (~1 min)")
+    mods = ctx.moduli()
+    alive, res = ctx.run([2])
+    say("50k-digit golden: round done")
+    for g, n, ok, r in zip(gold, mods, alive, res):
+        nw = g["n_words"]
+        assert hashlib.sha256(n.to_bytes(4 * nw, "little")).hexdigest() == g["n_sha256"]
+        if n % 2 == 0:
+            assert not ok
+            continue
+        assert ctx.s[mods.index(n)] == g["s"]
+        assert hashlib.sha256(r[0].to_bytes(4 * nw, "little")).hexdigest() == g["residue_sha256"]
+        assert ok == g["pass"]
+    ctx.close()
+
+
+def test_device_builder_matches_gmp_and_python(gpu):
+    """N, its decimal reversal and the reference's rev formula, built on the
+    device from (a, b, c, k, j), equal GMP's mpz construction (mpz_ui_pow_ui
+    / mul_ui / sub, and mpz_set_str of the reversed string)."""
+    from _support import gmp_build
+    import json
+    from _support import GOLDEN
+    cfg = json.loads((GOLDEN / "configs.json").read_text())
+    cases = [(5, 2, 1, 3, 5), (30, 15, 7, 12, 34), (31, 27, 5, 4096, 7), (60, 59, 1, 9, 1),
+             (40, 3, 1, 4096, 4096), (25, 24, 10, 9, 9), (2000, 1000, 100, 4095, 77)]
+    cases += [tuple(x) for x in cfg["3"]["candidates"][:6]]
+    cands = []
+    for (a, b, c, k, j) in cases:
+        for which in (0, 1, 2):
+            lk, lj = len(str(k)), len(str(j))
+            if which == 2 and (a - b - lk < 0 or a - c - lj < 0):
+                continue
+            cands.append((a, b, c, k, j, which))
+    ctx = gpu.MrContext(candidates=cands)
+    got = ctx.moduli()
+    for cd, g in zip(cands, got):
+        a, b, c, k, j, which = cd
+        assert g == gmp_build(a, b, c, k, j, which), cd
+        if a <= 2000:
+            n = 10 ** a - k * 10 ** b - j * 10 ** c - 1
+            want = n if which == 0 else int(str(n)[::-1]) if which == 1 else g
+            assert g == want
+    ctx.close()
+
+
+def test_even_and_tiny_moduli_are_decided(gpu):
+    ctx = gpu.MrContext([2, 3, 1, 4, 10 ** 30 + 8, 7, 2 ** 89 - 1])
+    alive, res = ctx.run([2, 3])
+    assert alive == [True, True, False, False, False, True, True]
     ctx.close()
