@@ -44,6 +44,18 @@ __host__ __device__ constexpr uint32_t neg_inv32(uint32_t p)
     return 0u - x;
 }
 
+// The thread index, read through an empty asm statement.  Every address a
+// thread derives from its index is loop-invariant, and the optimiser would
+// otherwise hoist all of them (LDS and global addresses, twiddle loads) out
+// of the squaring loop at once and spill registers; an opaque index makes
+// each use recompute its few address operations where it is needed.
+__device__ __forceinline__ int tid_()
+{
+    int t = (int)threadIdx.x;
+    asm volatile("" : "+v"(t));
+    return t;
+}
+
 // ---------------------------------------------------------------- mod p ops
 __device__ __forceinline__ uint32_t add_mod(uint32_t a, uint32_t b, uint32_t p)
 {
@@ -96,7 +108,7 @@ __device__ void ntt_fwd(uint32_t *X, const Twiddles &T, int q0)
 {
     constexpr int M = 1 << LOGM;
     constexpr int GROUPS = M / 4;
-    const int tid = threadIdx.x;
+    const int tid = tid_();
     int s = 0;
     for (; s + 1 < LOGM; s += 2) {
         const int h2 = M >> (s + 2);
@@ -146,7 +158,7 @@ __device__ void ntt_inv(uint32_t *X, const Twiddles &T, int q0)
 {
     constexpr int M = 1 << LOGM;
     constexpr int GROUPS = M / 4;
-    const int tid = threadIdx.x;
+    const int tid = tid_();
     int t = 0;
     if (LOGM & 1) {   // first radix-2 stage, half = 1, twiddle 1
 #pragma unroll
@@ -195,6 +207,223 @@ __device__ void ntt_inv(uint32_t *X, const Twiddles &T, int q0)
     }
 }
 
+// ------------------------------------------------- fused transform passes
+// The same transforms as ntt_fwd / ntt_inv, organised for throughput:
+//  * per-thread trip counts are compile-time constants;
+//  * in every pass whose butterfly half-width is <= THREADS a thread's
+//    position inside its block, hence its twiddles, is the same for all its
+//    groups, so the twiddles are loaded once per pass;
+//  * the last forward pass and the first inverse pass act on the same four
+//    consecutive elements (two for odd LOGM) with the constant twiddles 1 and
+//    w^(+-M/4), so they are fused with the pointwise step in registers.
+
+// Regular forward radix-4 pass starting at stage S.
+template <int LOGM, int S, int NP>
+__device__ __forceinline__ void dif_pass(uint32_t *X, const Twiddles &T, int q0)
+{
+    constexpr int M = 1 << LOGM, H2 = M >> (S + 2), GROUPS = M / 4;
+    constexpr int IT = GROUPS >= THREADS ? GROUPS / THREADS : 1;
+    const int tid = tid_();
+    if (GROUPS >= THREADS || tid < GROUPS) {
+#pragma unroll 1
+        for (int q = 0; q < NP; q++) {
+            uint32_t *Y = X + q * M;
+            const uint32_t p = prime(q0 + q);
+            const uint32_t *tw = T.tw(q0 + q), *twq = T.twq(q0 + q);
+            uint32_t wa = 0, waq = 0, wb = 0, wbq = 0, wc = 0, wcq = 0;
+            if constexpr (H2 <= THREADS) {
+                const int pos = tid & (H2 - 1);
+                const int ia = pos << S, ib = (pos + H2) << S, ic = pos << (S + 1);
+                wa = tw[ia]; waq = twq[ia]; wb = tw[ib]; wbq = twq[ib]; wc = tw[ic]; wcq = twq[ic];
+            }
+#pragma unroll
+            for (int it = 0; it < IT; it++) {
+                const int g = tid + it * THREADS;
+                const int pos = g & (H2 - 1);
+                if constexpr (H2 > THREADS) {
+                    const int ia = pos << S, ib = (pos + H2) << S, ic = pos << (S + 1);
+                    wa = tw[ia]; waq = twq[ia]; wb = tw[ib]; wbq = twq[ib]; wc = tw[ic]; wcq = twq[ic];
+                }
+                const int base = ((g - pos) << 2) + pos;
+                const uint32_t x0 = Y[base], x1 = Y[base + H2], x2 = Y[base + 2 * H2], x3 = Y[base + 3 * H2];
+                const uint32_t y0 = add_mod(x0, x2, p);
+                const uint32_t y2 = shoup(x0 - x2 + p, wa, waq, p);
+                const uint32_t y1 = add_mod(x1, x3, p);
+                const uint32_t y3 = shoup(x1 - x3 + p, wb, wbq, p);
+                Y[base] = add_mod(y0, y1, p);
+                Y[base + H2] = shoup(y0 - y1 + p, wc, wcq, p);
+                Y[base + 2 * H2] = add_mod(y2, y3, p);
+                Y[base + 3 * H2] = shoup(y2 - y3 + p, wc, wcq, p);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Regular inverse radix-4 pass starting at stage T0 (half 2^T0).
+template <int LOGM, int T0, int NP>
+__device__ __forceinline__ void dit_pass(uint32_t *X, const Twiddles &T, int q0)
+{
+    constexpr int M = 1 << LOGM, H = 1 << T0, GROUPS = M / 4;
+    constexpr int IT = GROUPS >= THREADS ? GROUPS / THREADS : 1;
+    const int tid = tid_();
+    if (GROUPS >= THREADS || tid < GROUPS) {
+#pragma unroll 1
+        for (int q = 0; q < NP; q++) {
+            uint32_t *Y = X + q * M;
+            const uint32_t p = prime(q0 + q);
+            const uint32_t *itw = T.itw(q0 + q), *itwq = T.itwq(q0 + q);
+            uint32_t wa = 0, waq = 0, wb = 0, wbq = 0, wc = 0, wcq = 0;
+            if constexpr (H <= THREADS) {
+                const int pos = tid & (H - 1);
+                const int ia = pos << (LOGM - 1 - T0), ib = pos << (LOGM - 2 - T0),
+                          ic = (pos + H) << (LOGM - 2 - T0);
+                wa = itw[ia]; waq = itwq[ia]; wb = itw[ib]; wbq = itwq[ib]; wc = itw[ic]; wcq = itwq[ic];
+            }
+#pragma unroll
+            for (int it = 0; it < IT; it++) {
+                const int g = tid + it * THREADS;
+                const int pos = g & (H - 1);
+                if constexpr (H > THREADS) {
+                    const int ia = pos << (LOGM - 1 - T0), ib = pos << (LOGM - 2 - T0),
+                              ic = (pos + H) << (LOGM - 2 - T0);
+                    wa = itw[ia]; waq = itwq[ia]; wb = itw[ib]; wbq = itwq[ib]; wc = itw[ic]; wcq = itwq[ic];
+                }
+                const int base = ((g - pos) << 2) + pos;
+                const uint32_t x0 = Y[base], x1 = Y[base + H], x2 = Y[base + 2 * H], x3 = Y[base + 3 * H];
+                const uint32_t b1 = shoup(x1, wa, waq, p);
+                const uint32_t b3 = shoup(x3, wa, waq, p);
+                const uint32_t y0 = add_mod(x0, b1, p), y1 = sub_mod(x0, b1, p);
+                const uint32_t y2 = add_mod(x2, b3, p), y3 = sub_mod(x2, b3, p);
+                const uint32_t c2 = shoup(y2, wb, wbq, p);
+                const uint32_t c3 = shoup(y3, wc, wcq, p);
+                Y[base] = add_mod(y0, c2, p);
+                Y[base + 2 * H] = sub_mod(y0, c2, p);
+                Y[base + H] = add_mod(y1, c3, p);
+                Y[base + 3 * H] = sub_mod(y1, c3, p);
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Forward passes S, S+2, ... while the pass is not the last one.
+template <int LOGM, int S, int NP>
+__device__ __forceinline__ void dif_regular(uint32_t *X, const Twiddles &T, int q0)
+{
+    constexpr int LAST = (LOGM & 1) ? LOGM - 1 : LOGM - 2;   // first stage of the final pass
+    if constexpr (S < LAST) {
+        dif_pass<LOGM, S, NP>(X, T, q0);
+        dif_regular<LOGM, S + 2, NP>(X, T, q0);
+    }
+}
+
+// Inverse passes after the first one.
+template <int LOGM, int T0, int NP>
+__device__ __forceinline__ void dit_regular(uint32_t *X, const Twiddles &T, int q0)
+{
+    if constexpr (T0 + 1 < LOGM) {
+        dit_pass<LOGM, T0, NP>(X, T, q0);
+        dit_regular<LOGM, T0 + 2, NP>(X, T, q0);
+    }
+}
+
+// Final forward pass (stages LOGM-2, LOGM-1, or the lone radix-2 stage),
+// optionally followed by the pointwise step and the first inverse pass.
+// MODE 0: forward only; MODE 1: pointwise square when B is null, else
+// times B (kept times 2^32) -- one code path, a uniform branch.
+template <int LOGM, int NP, int MODE>
+__device__ __forceinline__ void middle(uint32_t *X, const Twiddles &T, int q0, const uint32_t *B,
+                                       const uint32_t *np_tab)
+{
+    constexpr int M = 1 << LOGM;
+    const int tid = tid_();
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+        uint32_t *Y = X + q * M;
+        const uint32_t p = prime(q0 + q), np = np_tab[q0 + q];
+        const uint32_t *Bq = B ? B + (size_t)(q0 + q) * M : nullptr;
+        if constexpr ((LOGM & 1) == 0) {
+            constexpr int GROUPS = M / 4;
+            constexpr int IT = GROUPS >= THREADS ? GROUPS / THREADS : 1;
+            const uint32_t *tw = T.tw(q0 + q), *itw = T.itw(q0 + q);
+            const uint32_t w4 = tw[M / 4], w4q = tw[M / 2 + M / 4];     // tw, twq blocks
+            const uint32_t v4 = itw[M / 4], v4q = itw[M / 2 + M / 4];
+            if (GROUPS >= THREADS || tid < GROUPS) {
+#pragma unroll
+                for (int it = 0; it < IT; it++) {
+                    const int g = tid + it * THREADS;
+                    uint4 v = *reinterpret_cast<uint4 *>(Y + 4 * g);
+                    // DIF stages LOGM-2 (twiddles 1, w^(M/4)) and LOGM-1 (twiddle 1)
+                    const uint32_t y0 = add_mod(v.x, v.z, p), y2 = sub_mod(v.x, v.z, p);
+                    const uint32_t y1 = add_mod(v.y, v.w, p);
+                    const uint32_t y3 = shoup(v.y - v.w + p, w4, w4q, p);
+                    uint32_t z0 = add_mod(y0, y1, p), z1 = sub_mod(y0, y1, p);
+                    uint32_t z2 = add_mod(y2, y3, p), z3 = sub_mod(y2, y3, p);
+                    if constexpr (MODE != 0) {
+                        uint4 bb = make_uint4(z0, z1, z2, z3);
+                        if (Bq) bb = *reinterpret_cast<const uint4 *>(Bq + 4 * g);
+                        z0 = mont(z0, bb.x, p, np); z1 = mont(z1, bb.y, p, np);
+                        z2 = mont(z2, bb.z, p, np); z3 = mont(z3, bb.w, p, np);
+                        // DIT stages 0 (twiddle 1) and 1 (twiddles 1, w^-(M/4))
+                        const uint32_t a0 = add_mod(z0, z1, p), a1 = sub_mod(z0, z1, p);
+                        const uint32_t a2 = add_mod(z2, z3, p), a3 = sub_mod(z2, z3, p);
+                        const uint32_t c3 = shoup(a3, v4, v4q, p);
+                        v.x = add_mod(a0, a2, p);
+                        v.z = sub_mod(a0, a2, p);
+                        v.y = add_mod(a1, c3, p);
+                        v.w = sub_mod(a1, c3, p);
+                    } else {
+                        v.x = z0; v.y = z1; v.z = z2; v.w = z3;
+                    }
+                    *reinterpret_cast<uint4 *>(Y + 4 * g) = v;
+                }
+            }
+        } else {
+            constexpr int PAIRS = M / 2;
+            constexpr int IT = PAIRS >= THREADS ? PAIRS / THREADS : 1;
+            if (PAIRS >= THREADS || tid < PAIRS) {
+#pragma unroll
+                for (int it = 0; it < IT; it++) {
+                    const int g = tid + it * THREADS;
+                    uint2 v = *reinterpret_cast<uint2 *>(Y + 2 * g);
+                    uint32_t z0 = add_mod(v.x, v.y, p), z1 = sub_mod(v.x, v.y, p);
+                    if constexpr (MODE != 0) {
+                        uint2 bb = make_uint2(z0, z1);
+                        if (Bq) bb = *reinterpret_cast<const uint2 *>(Bq + 2 * g);
+                        z0 = mont(z0, bb.x, p, np); z1 = mont(z1, bb.y, p, np);
+                        v.x = add_mod(z0, z1, p);
+                        v.y = sub_mod(z0, z1, p);
+                    } else {
+                        v.x = z0; v.y = z1;
+                    }
+                    *reinterpret_cast<uint2 *>(Y + 2 * g) = v;
+                }
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// Cyclic convolution of the NP buffers with themselves (B null) or with the
+// stored transform B; output scaled by M (and by 2^-32 for the square).
+template <int LOGM, int NP>
+__device__ __forceinline__ void convolve(uint32_t *X, const Twiddles &T, int q0, const uint32_t *B,
+                                         const uint32_t *np_tab)
+{
+    dif_regular<LOGM, 0, NP>(X, T, q0);
+    middle<LOGM, NP, 1>(X, T, q0, B, np_tab);
+    dit_regular<LOGM, (LOGM & 1) ? 1 : 2, NP>(X, T, q0);
+}
+
+// Forward transform only (bit-reversed output).
+template <int LOGM, int NP>
+__device__ __forceinline__ void forward(uint32_t *X, const Twiddles &T, int q0, const uint32_t *np_tab)
+{
+    dif_regular<LOGM, 0, NP>(X, T, q0);
+    middle<LOGM, NP, 0>(X, T, q0, nullptr, np_tab);
+}
+
 // ------------------------------------------------------------- scans
 struct Scratch {            // LDS scratch beside the transform buffers
     uint64_t carry[THREADS];
@@ -208,7 +437,7 @@ struct Scratch {            // LDS scratch beside the transform buffers
 __device__ __forceinline__ uint32_t block_carry_in(bool g, bool pr, Scratch &sc)
 {
     const int lane = __lane_id();
-    const int w = threadIdx.x >> 6;
+    const int w = tid_() >> 6;
     const uint64_t G = __ballot(g), P = __ballot(pr);
     const uint64_t Y = G | P;
     const uint64_t s0 = G + Y;
@@ -226,7 +455,7 @@ __device__ __forceinline__ uint32_t block_carry_in(bool g, bool pr, Scratch &sc)
 
 __device__ __forceinline__ bool block_any(bool v, Scratch &sc)
 {
-    const int w = threadIdx.x >> 6;
+    const int w = tid_() >> 6;
     const uint64_t b = __ballot(v);
     if (__lane_id() == 0) sc.wave[2 * WAVES + 0 + 0] = 0u;
     __syncthreads();
@@ -255,7 +484,7 @@ template <int C, class Get0, class Get1>
 __device__ __forceinline__ void crt_carry(uint32_t (&v)[C], Get0 get0, Get1 get1,
                                           const CrtConst &k, int keep, Scratch &sc)
 {
-    const int t = threadIdx.x;
+    const int t = tid_();
     uint64_t acc = 0;
 #pragma unroll
     for (int i = 0; i < C; i++) {
@@ -302,7 +531,7 @@ template <int C>
 __device__ __forceinline__ void add_vec(uint32_t (&v)[C], const uint32_t (&w)[C], uint32_t cin0,
                                         Scratch &sc)
 {
-    uint32_t cc = threadIdx.x == 0 ? cin0 : 0u;
+    uint32_t cc = tid_() == 0 ? cin0 : 0u;
 #pragma unroll
     for (int i = 0; i < C; i++) {
         const uint32_t s = v[i] + w[i] + cc;
@@ -348,7 +577,7 @@ __device__ __forceinline__ bool sub_vec(uint32_t (&d)[C], const uint32_t (&v)[C]
         bb = (s >> 31) & 1u;
     }
     // borrow out of the top thread
-    if (threadIdx.x == THREADS - 1) sc.wave[2 * WAVES + 1] = bb;
+    if (tid_() == THREADS - 1) sc.wave[2 * WAVES + 1] = bb;
     __syncthreads();
     const bool r = sc.wave[2 * WAVES + 1] != 0u;
     __syncthreads();

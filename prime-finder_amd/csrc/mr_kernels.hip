@@ -74,7 +74,7 @@ struct Smem {
 template <int C>
 __device__ __forceinline__ void load_vec(uint32_t (&v)[C], const uint32_t *src, int len, int off)
 {
-    const int p0 = threadIdx.x * C;
+    const int p0 = tid_() * C;
 #pragma unroll
     for (int i = 0; i < C; i++) {
         const int idx = p0 + i - off;
@@ -85,7 +85,7 @@ __device__ __forceinline__ void load_vec(uint32_t (&v)[C], const uint32_t *src, 
 template <int C>
 __device__ __forceinline__ void store_vec(const uint32_t (&v)[C], uint32_t *dst, int len, int off)
 {
-    const int p0 = threadIdx.x * C;
+    const int p0 = tid_() * C;
 #pragma unroll
     for (int i = 0; i < C; i++) {
         const int idx = p0 + i - off;
@@ -96,7 +96,7 @@ __device__ __forceinline__ void store_vec(const uint32_t (&v)[C], uint32_t *dst,
 template <int C>
 __device__ __forceinline__ void zero_below(uint32_t (&v)[C], int lim)
 {
-    const int p0 = threadIdx.x * C;
+    const int p0 = tid_() * C;
 #pragma unroll
     for (int i = 0; i < C; i++)
         if (p0 + i < lim) v[i] = 0u;
@@ -108,7 +108,7 @@ template <int LOGM, int NPW>
 __device__ __forceinline__ void stage_in(uint32_t *X, const uint32_t (&v)[G<LOGM>::C], int off, int len)
 {
     constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
-    const int p0 = threadIdx.x * C;
+    const int p0 = tid_() * C;
     if (off != 0) {
 #pragma unroll
         for (int q = 0; q < NPW; q++)
@@ -145,9 +145,9 @@ __device__ __forceinline__ void pointwise(uint32_t *X, const uint32_t *B, int q0
         uint32_t *Y = X + q * M;
         if (B) {
             const uint32_t *Bq = B + (size_t)(q0 + q) * M;
-            for (int i = threadIdx.x; i < M; i += THREADS) Y[i] = mont(Y[i], Bq[i], p, np);
+            for (int i = tid_(); i < M; i += THREADS) Y[i] = mont(Y[i], Bq[i], p, np);
         } else {
-            for (int i = threadIdx.x; i < M; i += THREADS) { const uint32_t y = Y[i]; Y[i] = mont(y, y, p, np); }
+            for (int i = tid_(); i < M; i += THREADS) { const uint32_t y = Y[i]; Y[i] = mont(y, y, p, np); }
         }
     }
     __syncthreads();
@@ -162,29 +162,23 @@ __device__ void tproduct(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
                          const uint32_t *Bf, uint32_t (&out)[G<LOGM>::C], int keep)
 {
     constexpr int M = G<LOGM>::M, C = G<LOGM>::C, NP = G<LOGM>::NP;
-    const CrtConst &k = Bf ? K.mul : K.sq;
-    const int p0 = threadIdx.x * C;
+    const CrtConst k = Bf ? K.mul : K.sq;
+    const int p0 = tid_() * C;
     if constexpr (NP == 2) {
         stage_in<LOGM, 2>(sm.X, a, off, len);
-        ntt_fwd<LOGM, 2>(sm.X, T, 0);
-        pointwise<LOGM, 2>(sm.X, Bf, 0, K);
-        ntt_inv<LOGM, 2>(sm.X, T, 0);
+        convolve<LOGM, 2>(sm.X, T, 0, Bf, K.np);
         const uint32_t *X0 = sm.X, *X1 = sm.X + M;
         crt_carry<C>(out, [&](int i) { return X0[p0 + i]; }, [&](int i) { return X1[p0 + i]; },
                      k, keep, sm.sc);
     } else {
         uint32_t y0[C];
         stage_in<LOGM, 1>(sm.X, a, off, len);
-        ntt_fwd<LOGM, 1>(sm.X, T, 0);
-        pointwise<LOGM, 1>(sm.X, Bf, 0, K);
-        ntt_inv<LOGM, 1>(sm.X, T, 0);
+        convolve<LOGM, 1>(sm.X, T, 0, Bf, K.np);
 #pragma unroll
         for (int i = 0; i < C; i++) y0[i] = sm.X[p0 + i];
         __syncthreads();
         stage_in<LOGM, 1>(sm.X, a, off, len);
-        ntt_fwd<LOGM, 1>(sm.X, T, 1);
-        pointwise<LOGM, 1>(sm.X, Bf, 1, K);
-        ntt_inv<LOGM, 1>(sm.X, T, 1);
+        convolve<LOGM, 1>(sm.X, T, 1, Bf, K.np);
         const uint32_t *X1 = sm.X;
         crt_carry<C>(out, [&](int i) { return y0[i]; }, [&](int i) { return X1[p0 + i]; },
                      k, keep, sm.sc);
@@ -199,91 +193,104 @@ __device__ void to_fft(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
     constexpr int M = G<LOGM>::M, NP = G<LOGM>::NP;
     for (int q0 = 0; q0 < 2; q0 += NP) {
         stage_in<LOGM, NP>(sm.X, v, 0, len);
-        ntt_fwd<LOGM, NP>(sm.X, T, q0);
+        forward<LOGM, NP>(sm.X, T, q0, K.np);
 #pragma unroll
         for (int q = 0; q < NP; q++) {
             const uint32_t p = prime(q0 + q), np = K.np[q0 + q], r2 = K.r2[q0 + q];
-            for (int i = threadIdx.x; i < M; i += THREADS)
+            for (int i = tid_(); i < M; i += THREADS)
                 dst[(size_t)(q0 + q) * M + i] = mont(sm.X[q * M + i], r2, p, np);
         }
         __syncthreads();
     }
 }
 
-// r <- (r + s) mod N for r, s < N (high or low layout alike, N in the same
-// layout as nv).
+// r <- r - N when r >= N; N read from Nl at layout offset off.  r < 2N.
 template <int C>
-__device__ __forceinline__ void add_modN(uint32_t (&r)[C], const uint32_t (&s)[C],
-                                         const uint32_t (&nv)[C], Scratch &sc)
+__device__ __forceinline__ void reduce_once(uint32_t (&r)[C], const uint32_t *Nl, int L, int off,
+                                            Scratch &sc)
 {
-    add_vec<C>(r, s, 0u, sc);
-    uint32_t d[C];
-    const bool lt = sub_vec<C>(d, r, nv, sc);
+    uint32_t t[C], d[C];
+    load_vec<C>(t, Nl, L, off);
+    const bool lt = sub_vec<C>(d, r, t, sc);
     if (!lt) {
 #pragma unroll
         for (int i = 0; i < C; i++) r[i] = d[i];
     }
 }
 
-// r <- r * s mod N for r < N, 1 <= s < 2^24 (double-and-add).
+// r <- r * s mod N for r < N, 1 <= s < 2^24 (double-and-add of modular
+// additions).  The multiplicand waits in LDS (stash, C * THREADS words) so
+// that at most three limb vectors are live.
 template <int C>
-__device__ void mulsmall_modN(uint32_t (&r)[C], uint32_t s, const uint32_t (&nv)[C], Scratch &sc)
+__device__ void mulsmall_modN(uint32_t (&r)[C], uint32_t s, const uint32_t *Nl, int L, int off,
+                              uint32_t *stash, Scratch &sc)
 {
-    uint32_t base[C];
+    const int p0 = tid_() * C;
 #pragma unroll
-    for (int i = 0; i < C; i++) base[i] = r[i];
+    for (int i = 0; i < C; i++) stash[p0 + i] = r[i];       // own slots only: no barrier
     const int top = 31 - __clz((int)s);
     for (int b = top - 1; b >= 0; b--) {
-        uint32_t cp[C];
+        add_vec<C>(r, r, 0u, sc);                            // 2r < 2N
+        reduce_once<C>(r, Nl, L, off, sc);
+        if ((s >> b) & 1u) {
+            uint32_t t[C];
 #pragma unroll
-        for (int i = 0; i < C; i++) cp[i] = r[i];
-        add_modN<C>(r, cp, nv, sc);
-        if ((s >> b) & 1u) add_modN<C>(r, base, nv, sc);
+            for (int i = 0; i < C; i++) t[i] = stash[p0 + i];
+            add_vec<C>(r, t, 0u, sc);
+            reduce_once<C>(r, Nl, L, off, sc);
+        }
     }
 }
 
 // Montgomery: r <- A * B * R^-1 mod N, output in high layout (positions
-// [L, 2L]).  A is positions [off, off+L) of r; B as in tproduct.
+// [L, 2L]).  A is positions [off, off+L) of r; B as in tproduct.  The high
+// half of T = A*B waits in the candidate's scratch area while m and m*N are
+// formed, so only two limb vectors are live across a transform.
 template <int LOGM>
 __device__ void montmul(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, const Dev &D, int c,
                         uint32_t (&r)[G<LOGM>::C], int off, const uint32_t *Bf)
 {
     constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
     const int L = D.L;
-    uint32_t t[C], m[C];
-    tproduct<LOGM>(sm, T, K, r, off, L, Bf, t, M);             // t = A*B, < R^2
-    bool nz = false;
-    const int p0 = threadIdx.x * C;
-#pragma unroll
-    for (int i = 0; i < C; i++) nz |= (p0 + i < L) && t[i] != 0u;
-    const bool cbit = block_any(nz, sm.sc);
+    const int p0 = tid_() * C;
+    uint32_t *park = D.scr + (size_t)c * 2 * M + M;            // 2nd half of the scratch
+    const uint32_t *Nl = D.N + (size_t)c * D.stride;
     const uint32_t *Npf = D.Npf + (size_t)c * 2 * M;
     const uint32_t *Nf = D.Nf + (size_t)c * 2 * M;
-    tproduct<LOGM>(sm, T, K, t, 0, L, Npf, m, L);             // m = t_lo N' mod R
-    tproduct<LOGM>(sm, T, K, m, 0, L, Nf, m, M);              // m = m N
-    zero_below<C>(t, L);
-    zero_below<C>(m, L);
-    add_vec<C>(t, m, 0u, sm.sc);                              // T_hi + (mN)_hi, shifted by L
-    // + [T_lo != 0] at position L
+    bool cbit = false;
+    // stage 0: r = T = A*B (< R^2); stage 1: r = m = T_lo N' mod R;
+    // stage 2: r = m N.  One product instance keeps the code (and its
+    // register allocation) single.
+#pragma unroll 1
+    for (int stage = 0; stage < 3; stage++) {
+        const uint32_t *B = stage == 0 ? Bf : stage == 1 ? Npf : Nf;
+        tproduct<LOGM>(sm, T, K, r, stage == 0 ? off : 0, L, B, r, stage == 1 ? L : M);
+        if (stage == 0) {
+            bool nz = false;
+#pragma unroll
+            for (int i = 0; i < C; i++) nz |= (p0 + i < L) && r[i] != 0u;
+            cbit = block_any(nz, sm.sc);
+            store_vec<C>(r, park, L + 1, L);                   // T_hi
+        }
+    }
+    zero_below<C>(r, L);
     {
-        uint32_t one[C];
+        uint32_t t[C];
+        load_vec<C>(t, park, L + 1, L);
+        // + [T_lo != 0] at position L: T_hi's lowest limb is below 2^24, so
+        // adding 1 cannot overflow the 32-bit lane before add_vec normalises
 #pragma unroll
-        for (int i = 0; i < C; i++) one[i] = (cbit && p0 + i == L) ? 1u : 0u;
-        add_vec<C>(t, one, 0u, sm.sc);
+        for (int i = 0; i < C; i++) t[i] += (cbit && p0 + i == L) ? 1u : 0u;
+        add_vec<C>(r, t, 0u, sm.sc);                           // T_hi + (mN)_hi + c
     }
-    load_vec<C>(m, D.N + (size_t)c * D.stride, L, L);         // N, high layout
-    const bool lt = sub_vec<C>(r, t, m, sm.sc);
-    if (lt) {
-#pragma unroll
-        for (int i = 0; i < C; i++) r[i] = t[i];
-    }
+    reduce_once<C>(r, Nl, L, L, sm.sc);
 }
 
 // x <- x * s (plain integer product), 1 <= s < 2^24.
 template <int C>
 __device__ void mul_small_vec(uint32_t (&x)[C], uint32_t s, Scratch &sc)
 {
-    const int t = threadIdx.x;
+    const int t = tid_();
     uint64_t acc = 0;
 #pragma unroll
     for (int i = 0; i < C; i++) {
@@ -317,7 +324,7 @@ __device__ void mul_small_vec(uint32_t (&x)[C], uint32_t s, Scratch &sc)
 template <int C>
 __device__ void shl_vec(uint32_t (&x)[C], long e, uint32_t *X, int M)
 {
-    const int p0 = threadIdx.x * C;
+    const int p0 = tid_() * C;
 #pragma unroll
     for (int i = 0; i < C; i++) X[p0 + i] = x[i];
     __syncthreads();
@@ -457,7 +464,7 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
     // bit length and two-adic valuation of N - 1
     {
         int top = -1, low = 1 << 30;
-        const int p0 = threadIdx.x * C;
+        const int p0 = tid_() * C;
 #pragma unroll
         for (int i = 0; i < C; i++) {
             if (nv[i]) top = max(top, (p0 + i) * 24 + 31 - __clz((int)nv[i]));
@@ -503,7 +510,7 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
             for (int i = 0; i < C; i++) two[i] = (threadIdx.x == 0 && i == 0) ? 2u : 0u;
             add_vec<C>(e, two, 0u, sm.sc);
             {   // truncate to R
-                const int p0 = threadIdx.x * C;
+                const int p0 = tid_() * C;
 #pragma unroll
                 for (int i = 0; i < C; i++) if (p0 + i >= L) e[i] = 0u;
             }
@@ -516,16 +523,14 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
     // R mod N: 2^nbits - N, then doubled (24 L - nbits) times mod N.
     {
         uint32_t pw[C];
-        const int p0 = threadIdx.x * C;
+        const int p0 = tid_() * C;
 #pragma unroll
         for (int i = 0; i < C; i++)
             pw[i] = (p0 + i == nbits / 24) ? (1u << (nbits % 24)) : 0u;
         sub_vec<C>(y, pw, nv, sm.sc);                  // 2^nbits - N  (< N)
         for (int k = nbits; k < 24 * L; k++) {
-            uint32_t cp[C];
-#pragma unroll
-            for (int i = 0; i < C; i++) cp[i] = y[i];
-            add_modN<C>(y, cp, nv, sm.sc);
+            add_vec<C>(y, y, 0u, sm.sc);
+            reduce_once<C>(y, Nl, L, 0, sm.sc);
         }
         store_vec<C>(y, D.one + (size_t)c * D.stride, D.stride, 0);
         sub_vec<C>(e, nv, y, sm.sc);                   // N - (R mod N) = (N-1) R mod N
@@ -540,12 +545,12 @@ __global__ __launch_bounds__(THREADS) void round_start_kernel(Dev D, uint32_t ba
 {
     constexpr int C = G<LOGM>::C;
     __shared__ Scratch sc;
+    __shared__ uint32_t stash[G<LOGM>::M];
     const int c = blockIdx.x;
     if (!D.alive[c] || D.pre[c]) return;
-    uint32_t r[C], nv[C];
-    load_vec<C>(nv, D.N + (size_t)c * D.stride, D.L, 0);
+    uint32_t r[C];
     load_vec<C>(r, D.one + (size_t)c * D.stride, D.L, 0);
-    if (base > 1u) mulsmall_modN<C>(r, base, nv, sc);
+    if (base > 1u) mulsmall_modN<C>(r, base, D.N + (size_t)c * D.stride, D.L, 0, stash, sc);
     store_vec<C>(r, D.res + (size_t)c * D.stride, D.stride, 0);
     if (threadIdx.x == 0) D.bitpos[c] = D.nbits[c] - 2;
 }
@@ -565,13 +570,12 @@ __global__ __launch_bounds__(THREADS) void exp_kernel(Dev D, Twiddles T, Consts 
     if (bp < s) return;
     const int L = D.L;
     const uint32_t *Nl = D.N + (size_t)c * D.stride;
-    uint32_t r[C], nv[C];
+    uint32_t r[C];
     load_vec<C>(r, D.res + (size_t)c * D.stride, L, L);       // high layout
-    load_vec<C>(nv, Nl, L, L);
     for (int k = 0; k < steps && bp >= s; k++, bp--) {
         montmul<LOGM>(sm, T, K, D, c, r, L, nullptr);
         const uint32_t bit = (Nl[bp / 24] >> (bp % 24)) & 1u;
-        if (bit) mulsmall_modN<C>(r, base, nv, sm.sc);
+        if (bit) mulsmall_modN<C>(r, base, Nl, L, L, sm.X, sm.sc);
     }
     store_vec<C>(r, D.res + (size_t)c * D.stride, L, L);
     if (threadIdx.x == 0) D.bitpos[c] = bp;
@@ -647,9 +651,7 @@ __global__ __launch_bounds__(THREADS) void debug_kernel(Dev D, Twiddles T, Const
         montmul<LOGM>(sm, T, K, D, c, a, 0, scr);
         store_vec<C>(a, out + (size_t)c * stride2, L, L);
     } else {
-        uint32_t nv[C];
-        load_vec<C>(nv, D.N + (size_t)c * D.stride, L, 0);
-        mulsmall_modN<C>(a, s, nv, sm.sc);
+        mulsmall_modN<C>(a, s, D.N + (size_t)c * D.stride, L, 0, sm.X, sm.sc);
         store_vec<C>(a, out + (size_t)c * stride2, L, 0);
     }
 }
