@@ -33,10 +33,11 @@ extern "C" {
 typedef struct mr_ctx mr_ctx_t;
 
 /* Build a context from n_batch moduli, each n_words little-endian uint32
- * words, laid out [n_batch][n_words].  Bit length <= 196 000 in this build
- * (transforms up to 2^14).  Even moduli and 0, 1 are reported composite and
- * 2, 3 prime without running rounds.  device = HIP device index.  NULL on
- * error.
+ * words, laid out [n_batch][n_words].  Bit length <= 384 000 (24-bit limbs,
+ * transforms of 2^11 .. 2^15 points; 50 000 digits use 2^14, 100 000
+ * digits 2^15).  Even moduli and 0, 1 are reported composite and
+ * 2, 3 prime without running rounds.  device = HIP device index, or -1
+ * for the calling thread's current device.  NULL on error.
  * Replaces BatchModCtx(N_all, n_limbs, n_batch, device)
  * (batch_mod_ctx.cuh:132-133). */
 mr_ctx_t *mr_ctx_create(const uint32_t *N_words, int n_words, int n_batch, int device);
@@ -77,6 +78,71 @@ int mr_ctx_info(const mr_ctx_t *ctx, int *n_words, int *limbs, int *log_len, int
 int mr_ctx_moduli(const mr_ctx_t *ctx, uint32_t *out);
 
 void mr_ctx_destroy(mr_ctx_t *ctx);
+
+/* ---- Continuous batching of candidates of mixed sizes ----------------
+ *
+ * A scheduler tests (a, b, c, k, j, which) candidates against a fixed list
+ * of bases, retiring each at its first failing base.  Candidates are sorted
+ * into size buckets by transform length (2^11 .. 2^15 points); each bucket
+ * owns a pool of slots in HBM (one workgroup each).  Squarings run in chunks
+ * of `chunk` steps over every running slot on one stream, while a second
+ * stream ends rounds, starts the next base and builds newly admitted
+ * candidates into freed slots, so the GPU never waits for admission.  At
+ * most `max_active` slots (0: the GPU's compute-unit count) are in use at
+ * once over all buckets; the slot memory of all pools together stays within
+ * `hbm_budget` bytes (0: 64 GiB).  Admission is first come, first served.
+ *
+ * No counterpart in the reference: its runner tests one fixed batch of equal
+ * size per call (miller_rabin_runner.cu:281-473) and keeps squaring
+ * candidates that already failed (:346-353).
+ *
+ * Not thread-safe: one host thread drives a scheduler. */
+typedef struct mr_sched mr_sched_t;
+
+typedef struct {
+    int64_t tag;          /* the caller's id from mr_sched_submit          */
+    int32_t verdict;      /* 1 = strong probable prime to every base, 0 = composite */
+    int32_t rounds;       /* bases actually run (early retirement)         */
+    double t_submit;      /* CLOCK_MONOTONIC seconds                       */
+    double t_admit;       /* built into a slot                             */
+    double t_done;        /* verdict seen by the host                      */
+} mr_result_t;
+
+typedef struct {
+    int64_t submitted, completed, pending, active;
+    int64_t chunks;       /* squaring-chunk launches                       */
+    int64_t admin_batches;
+    int64_t squarings;    /* slot-steps issued (upper bound of work done)  */
+    int64_t hbm_bytes;    /* slot memory allocated                         */
+    int32_t max_active, n_pools;
+} mr_sched_stats_t;
+
+mr_sched_t *mr_sched_create(const uint32_t *bases, int n_bases, int device,
+                            int max_active, int chunk, int64_t hbm_budget);
+/* Queue n candidates (validated first; -1 and nothing queued if any cannot
+ * be formed or is too large). */
+int mr_sched_submit(mr_sched_t *s, const mr_candidate_t *cands, const int64_t *tags, int n);
+/* Advance the scheduler.  block = 0: launch what can be launched and
+ * return; block = 1: also wait for the next chunk or admission batch to
+ * finish.  Returns the number of candidates not yet decided (>= 0), or -2
+ * on a HIP error. */
+int mr_sched_pump(mr_sched_t *s, int block);
+/* Move up to cap finished results to out; returns how many. */
+int mr_sched_results(mr_sched_t *s, mr_result_t *out, int cap);
+int mr_sched_stats(const mr_sched_t *s, mr_sched_stats_t *out);
+void mr_sched_destroy(mr_sched_t *s);
+
+/* One call over a whole list: verdict[i] (1 / 0) and rounds[i] (nullable)
+ * for cands[i].  Returns 0, -1 (invalid candidate) or -2 (HIP error). */
+int mr_test_batch(const mr_candidate_t *cands, int n, const uint32_t *bases, int n_bases,
+                  int device, uint8_t *verdict, int32_t *rounds);
+
+/* Smallest transform length (log2, 11..15) for contexts created afterwards;
+ * a context uses the smallest length >= this that holds its products.  The
+ * size-bucket tuner and the tests use it; process-wide, not thread-safe.
+ * No counterpart in the reference (its length is fixed at configure time,
+ * params.cmake). */
+int mr_set_min_log_len(int log_len);
 
 /* Benchmark helper (no counterpart in the reference, whose runner prints
  * per-phase timings itself, miller_rabin_runner.cu:205-275): starts a round

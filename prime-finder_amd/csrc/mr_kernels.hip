@@ -23,6 +23,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -32,9 +33,14 @@ namespace mr {
 // ------------------------------------------------------------------ device
 
 struct Dev {                    // per-batch device state (kernel argument)
-    int L;                      // limbs per number (R = 2^(24 L)), 2L < M
+    int L;                      // largest limb count of the context, 2L < M
     int stride;                 // words per linear limb vector (>= L + 1)
-    int n;                      // candidates
+    int n;                      // candidate slots
+    const int32_t *list;        // launch index -> slot (nullptr: identity)
+    int32_t *Ls;                // [n] limbs of the slot's number (R = 2^(24 Ls))
+    uint32_t *base;             // [n] base of the current round
+    int32_t *bidx;              // [n] index of that base in the scheduler's list
+    uint8_t *done;              // [n] scheduler: 1 = verdict final (in alive)
     uint32_t *N;                // [n][stride] modulus limbs
     uint32_t *Nf;               // [n][2M] NTT(N) * 2^32 per prime
     uint32_t *Npf;              // [n][2M] NTT(N') * 2^32 per prime
@@ -42,6 +48,7 @@ struct Dev {                    // per-batch device state (kernel argument)
     uint32_t *nm1;              // [n][stride] (N - 1) R mod N
     uint32_t *res;              // [n][stride] current residue (Montgomery form)
     uint32_t *scr;              // [n][2M] transform scratch
+    uint32_t *ysv;              // [n][M] first prime's product while the second runs (M = 2^15)
     int32_t *nbits;             // [n]
     int32_t *sval;              // [n] s with N - 1 = 2^s d
     int32_t *bitpos;            // [n] next exponent bit (bit index of N), < s when done
@@ -49,6 +56,11 @@ struct Dev {                    // per-batch device state (kernel argument)
     uint8_t *pre;               // [n] 0 = test, 1 = prime (2, 3), 2 = composite (even, < 2)
     uint32_t *err;              // [1] error latch
 };
+
+__device__ __forceinline__ int slot_id(const Dev &D)
+{
+    return D.list ? D.list[blockIdx.x] : (int)blockIdx.x;
+}
 
 struct Consts {
     CrtConst sq, mul;           // CRT scales after Montgomery square / exact product
@@ -155,11 +167,13 @@ __device__ __forceinline__ void pointwise(uint32_t *X, const uint32_t *B, int q0
 
 // out = A * B as normalised limbs (positions >= keep zeroed).  A is
 // positions [off, off+len) of a; B is a (square) when Bf == nullptr, else the
-// precomputed transform Bf.  a and out may alias.
+// precomputed transform Bf.  a and out may alias.  With one prime per LDS
+// pass (M = 2^15) the first prime's result waits in ysave (M words).
 template <int LOGM>
 __device__ void tproduct(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
                          const uint32_t (&a)[G<LOGM>::C], int off, int len,
-                         const uint32_t *Bf, uint32_t (&out)[G<LOGM>::C], int keep)
+                         const uint32_t *Bf, uint32_t (&out)[G<LOGM>::C], int keep,
+                         uint32_t *ysave)
 {
     constexpr int M = G<LOGM>::M, C = G<LOGM>::C, NP = G<LOGM>::NP;
     const CrtConst k = Bf ? K.mul : K.sq;
@@ -171,16 +185,15 @@ __device__ void tproduct(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
         crt_carry<C>(out, [&](int i) { return X0[p0 + i]; }, [&](int i) { return X1[p0 + i]; },
                      k, keep, sm.sc);
     } else {
-        uint32_t y0[C];
         stage_in<LOGM, 1>(sm.X, a, off, len);
         convolve<LOGM, 1>(sm.X, T, 0, Bf, K.np);
 #pragma unroll
-        for (int i = 0; i < C; i++) y0[i] = sm.X[p0 + i];
+        for (int i = 0; i < C; i++) ysave[p0 + i] = sm.X[p0 + i];
         __syncthreads();
         stage_in<LOGM, 1>(sm.X, a, off, len);
         convolve<LOGM, 1>(sm.X, T, 1, Bf, K.np);
         const uint32_t *X1 = sm.X;
-        crt_carry<C>(out, [&](int i) { return y0[i]; }, [&](int i) { return X1[p0 + i]; },
+        crt_carry<C>(out, [&](int i) { return ysave[p0 + i]; }, [&](int i) { return X1[p0 + i]; },
                      k, keep, sm.sc);
     }
 }
@@ -251,9 +264,10 @@ __device__ void montmul(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, cons
                         uint32_t (&r)[G<LOGM>::C], int off, const uint32_t *Bf)
 {
     constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
-    const int L = D.L;
+    const int L = D.Ls[c];
     const int p0 = tid_() * C;
     uint32_t *park = D.scr + (size_t)c * 2 * M + M;            // 2nd half of the scratch
+    uint32_t *ysave = D.ysv + (G<LOGM>::NP == 1 ? (size_t)c * M : 0);
     const uint32_t *Nl = D.N + (size_t)c * D.stride;
     const uint32_t *Npf = D.Npf + (size_t)c * 2 * M;
     const uint32_t *Nf = D.Nf + (size_t)c * 2 * M;
@@ -264,7 +278,7 @@ __device__ void montmul(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, cons
 #pragma unroll 1
     for (int stage = 0; stage < 3; stage++) {
         const uint32_t *B = stage == 0 ? Bf : stage == 1 ? Npf : Nf;
-        tproduct<LOGM>(sm, T, K, r, stage == 0 ? off : 0, L, B, r, stage == 1 ? L : M);
+        tproduct<LOGM>(sm, T, K, r, stage == 0 ? off : 0, L, B, r, stage == 1 ? L : M, ysave);
         if (stage == 0) {
             bool nz = false;
 #pragma unroll
@@ -343,7 +357,7 @@ __device__ void shl_vec(uint32_t (&x)[C], long e, uint32_t *X, int M)
 // x <- 5^e.
 template <int LOGM>
 __device__ void pow5(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, uint32_t (&x)[G<LOGM>::C],
-                     long e)
+                     long e, uint32_t *ysave)
 {
     constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
 #pragma unroll
@@ -352,27 +366,30 @@ __device__ void pow5(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, uint32_
     if (e <= 1) return;
     int top = 62 - __clzll((long long)e);      // bit below the leading one
     for (int b = top; b >= 0; b--) {
-        tproduct<LOGM>(sm, T, K, x, 0, M / 2, nullptr, x, M);
+        tproduct<LOGM>(sm, T, K, x, 0, M / 2, nullptr, x, M, ysave);
         if ((e >> b) & 1) mul_small_vec<C>(x, 5u, sm.sc);
     }
 }
 
 // N = sum of coef * 10^exp over the candidate's terms; one workgroup each.
+// Ls_in (optional, per launch index) sets the slot's limb count.
 template <int LOGM>
 __global__ __launch_bounds__(THREADS) void build_kernel(Dev D, Twiddles T, Consts K,
-                                                        const int64_t *terms, int tmax)
+                                                        const int64_t *terms, int tmax,
+                                                        const int32_t *Ls_in)
 {
     constexpr int C = G<LOGM>::C;
     __shared__ Smem<LOGM> sm;
-    const int c = blockIdx.x;
+    const int c = slot_id(D);
     uint32_t P[C], Q[C], x[C];
 #pragma unroll
     for (int i = 0; i < C; i++) P[i] = Q[i] = 0u;
-    const int64_t *tc = terms + (size_t)c * tmax * 2;
+    const int64_t *tc = terms + (size_t)blockIdx.x * tmax * 2;
+    if (Ls_in && threadIdx.x == 0) D.Ls[c] = Ls_in[blockIdx.x];
     for (int t = 0; t < tmax; t++) {
         const int64_t coef = tc[2 * t], ex = tc[2 * t + 1];
         if (coef == 0) continue;
-        pow5<LOGM>(sm, T, K, x, (long)ex);
+        pow5<LOGM>(sm, T, K, x, (long)ex, D.ysv + (G<LOGM>::NP == 1 ? (size_t)c * G<LOGM>::M : 0));
         const uint32_t mag = (uint32_t)(coef < 0 ? -coef : coef);
         if (mag != 1u) mul_small_vec<C>(x, mag, sm.sc);
         shl_vec<C>(x, (long)ex, sm.X, G<LOGM>::M);
@@ -381,6 +398,12 @@ __global__ __launch_bounds__(THREADS) void build_kernel(Dev D, Twiddles T, Const
     }
     sub_vec<C>(x, P, Q, sm.sc);
     store_vec<C>(x, D.N + (size_t)c * D.stride, D.stride, 0);
+}
+
+__global__ void fill_i32_kernel(int32_t *p, int n, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 __global__ void init_alive_kernel(Dev D)
@@ -455,8 +478,8 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
 {
     constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
     __shared__ Smem<LOGM> sm;
-    const int c = blockIdx.x;
-    const int L = D.L;
+    const int c = slot_id(D);
+    const int L = D.Ls[c];
     const uint32_t *Nl = D.N + (size_t)c * D.stride;
     uint32_t nv[C], y[C], e[C];
     load_vec<C>(nv, Nl, L, 0);
@@ -493,6 +516,7 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
     uint32_t *Nf = D.Nf + (size_t)c * 2 * M;
     uint32_t *Npf = D.Npf + (size_t)c * 2 * M;
     uint32_t *scr = D.scr + (size_t)c * 2 * M;
+    uint32_t *ysv = D.ysv + (G<LOGM>::NP == 1 ? (size_t)c * M : 0);
     to_fft<LOGM>(sm, T, K, nv, L, Nf);
 
     // N' = -N^-1 mod R by Newton: y <- y (2 + N y), doubling precision.
@@ -504,7 +528,7 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
         for (int i = 0; i < C; i++) y[i] = 0u;
         if (threadIdx.x == 0) y[0] = (0u - inv) & LIMB_MASK;
         for (int prec = 1; prec < L; prec *= 2) {
-            tproduct<LOGM>(sm, T, K, y, 0, L, Nf, e, L);     // e = N y mod R
+            tproduct<LOGM>(sm, T, K, y, 0, L, Nf, e, L, ysv);     // e = N y mod R
             uint32_t two[C];
 #pragma unroll
             for (int i = 0; i < C; i++) two[i] = (threadIdx.x == 0 && i == 0) ? 2u : 0u;
@@ -515,7 +539,7 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
                 for (int i = 0; i < C; i++) if (p0 + i >= L) e[i] = 0u;
             }
             to_fft<LOGM>(sm, T, K, e, L, scr);
-            tproduct<LOGM>(sm, T, K, y, 0, L, scr, y, L);    // y = y (2 + N y) mod R
+            tproduct<LOGM>(sm, T, K, y, 0, L, scr, y, L, ysv);    // y = y (2 + N y) mod R
         }
     }
     to_fft<LOGM>(sm, T, K, y, L, Npf);
@@ -540,17 +564,46 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
 
 // Start a round: r = base * R mod N, bitpos = nbits - 2 (the top bit of d
 // is 1 and is consumed by the initial value).
+// mode 0: every alive slot, with `base`.
+// mode 1 (scheduler, after setup): alive = the number needs rounds; first
+//   base of `bases`.
+// mode 2 (scheduler, after round_end): next base of `bases` if the slot is
+//   alive and one is left, else the verdict is final (done = 1).
 template <int LOGM>
-__global__ __launch_bounds__(THREADS) void round_start_kernel(Dev D, uint32_t base)
+__global__ __launch_bounds__(THREADS) void round_start_kernel(Dev D, uint32_t base, int mode,
+                                                              const uint32_t *bases, int n_bases)
 {
     constexpr int C = G<LOGM>::C;
     __shared__ Scratch sc;
     __shared__ uint32_t stash[G<LOGM>::M];
-    const int c = blockIdx.x;
-    if (!D.alive[c] || D.pre[c]) return;
+    __shared__ uint32_t go;
+    const int c = slot_id(D);
+    if (mode != 0) {
+        if (threadIdx.x == 0) {
+            int bi = 0;
+            bool run;
+            if (mode == 1) {
+                D.alive[c] = D.pre[c] == 2 ? 0 : 1;
+                run = D.pre[c] == 0 && n_bases > 0;
+            } else {
+                bi = D.bidx[c] + 1;
+                run = D.alive[c] && bi < n_bases;
+            }
+            if (run) { D.bidx[c] = bi; D.base[c] = bases[bi]; }
+            D.done[c] = run ? 0 : 1;
+            go = run;
+        }
+        __syncthreads();
+        if (!go) return;
+        base = D.base[c];
+    } else {
+        if (!D.alive[c] || D.pre[c]) return;
+        if (threadIdx.x == 0) D.base[c] = base;
+    }
+    const int L = D.Ls[c];
     uint32_t r[C];
-    load_vec<C>(r, D.one + (size_t)c * D.stride, D.L, 0);
-    if (base > 1u) mulsmall_modN<C>(r, base, D.N + (size_t)c * D.stride, D.L, 0, stash, sc);
+    load_vec<C>(r, D.one + (size_t)c * D.stride, L, 0);
+    if (base > 1u) mulsmall_modN<C>(r, base, D.N + (size_t)c * D.stride, L, 0, stash, sc);
     store_vec<C>(r, D.res + (size_t)c * D.stride, D.stride, 0);
     if (threadIdx.x == 0) D.bitpos[c] = D.nbits[c] - 2;
 }
@@ -558,17 +611,17 @@ __global__ __launch_bounds__(THREADS) void round_start_kernel(Dev D, uint32_t ba
 // Up to `steps` exponent bits: r = r^2, times base when the bit of d is set.
 // Bit i of d is bit i + s of N (N odd, so N - 1 differs only in bit 0).
 template <int LOGM>
-__global__ __launch_bounds__(THREADS) void exp_kernel(Dev D, Twiddles T, Consts K, uint32_t base,
-                                                      int steps)
+__global__ __launch_bounds__(THREADS) void exp_kernel(Dev D, Twiddles T, Consts K, int steps)
 {
     constexpr int C = G<LOGM>::C;
     __shared__ Smem<LOGM> sm;
-    const int c = blockIdx.x;
+    const int c = slot_id(D);
     if (!D.alive[c] || D.pre[c]) return;
     const int s = D.sval[c];
     int bp = D.bitpos[c];
     if (bp < s) return;
-    const int L = D.L;
+    const uint32_t base = D.base[c];
+    const int L = D.Ls[c];
     const uint32_t *Nl = D.N + (size_t)c * D.stride;
     uint32_t r[C];
     load_vec<C>(r, D.res + (size_t)c * D.stride, L, L);       // high layout
@@ -579,6 +632,20 @@ __global__ __launch_bounds__(THREADS) void exp_kernel(Dev D, Twiddles T, Consts 
     }
     store_vec<C>(r, D.res + (size_t)c * D.stride, L, L);
     if (threadIdx.x == 0) D.bitpos[c] = bp;
+}
+
+template <int C>
+__device__ __forceinline__ bool vec_equal_mem(const uint32_t (&a)[C], const uint32_t *src, int len,
+                                              int off, Scratch &sc)
+{
+    bool ne = false;
+    const int p0 = tid_() * C;
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+        const int idx = p0 + i - off;
+        ne |= a[i] != ((idx >= 0 && idx < len) ? src[idx] : 0u);
+    }
+    return !block_any(ne, sc);
 }
 
 template <int C>
@@ -598,31 +665,31 @@ __global__ __launch_bounds__(THREADS) void round_end_kernel(Dev D, Twiddles T, C
 {
     constexpr int C = G<LOGM>::C;
     __shared__ Smem<LOGM> sm;
-    const int c = blockIdx.x;
+    const int c = slot_id(D);
     if (!D.alive[c] || D.pre[c]) return;
-    const int L = D.L;
-    uint32_t r[C], a[C], b[C];
+    const int L = D.Ls[c];
+    const uint32_t *one = D.one + (size_t)c * D.stride;
+    const uint32_t *nm1 = D.nm1 + (size_t)c * D.stride;
+    uint32_t r[C];
     load_vec<C>(r, D.res + (size_t)c * D.stride, L, L);
-    load_vec<C>(a, D.one + (size_t)c * D.stride, L, L);
-    load_vec<C>(b, D.nm1 + (size_t)c * D.stride, L, L);
-    bool pass = vec_equal<C>(r, a, sm.sc) || vec_equal<C>(r, b, sm.sc);
+    bool pass = vec_equal_mem<C>(r, one, L, L, sm.sc) || vec_equal_mem<C>(r, nm1, L, L, sm.sc);
     if (out) {   // a^d mod N = REDC(r): product with 1
-        uint32_t x[C];
-#pragma unroll
-        for (int i = 0; i < C; i++) x[i] = r[i];
         uint32_t *one_f = D.scr + (size_t)c * 2 * G<LOGM>::M;
-        uint32_t unit[C];
+        {
+            uint32_t unit[C];
 #pragma unroll
-        for (int i = 0; i < C; i++) unit[i] = (threadIdx.x == 0 && i == 0) ? 1u : 0u;
-        to_fft<LOGM>(sm, T, K, unit, L, one_f);
-        montmul<LOGM>(sm, T, K, D, c, x, L, one_f);
-        store_vec<C>(x, out + (size_t)c * out_stride, L, L);
+            for (int i = 0; i < C; i++) unit[i] = (tid_() == 0 && i == 0) ? 1u : 0u;
+            to_fft<LOGM>(sm, T, K, unit, L, one_f);
+        }
+        montmul<LOGM>(sm, T, K, D, c, r, L, one_f);
+        store_vec<C>(r, out + (size_t)c * out_stride, L, L);
+        load_vec<C>(r, D.res + (size_t)c * D.stride, L, L);
     }
     const int s = D.sval[c];
     for (int i = 1; i < s && !pass; i++) {
         montmul<LOGM>(sm, T, K, D, c, r, L, nullptr);
-        pass = vec_equal<C>(r, b, sm.sc);
-        if (!pass && vec_equal<C>(r, a, sm.sc)) break;     // reached 1: composite
+        pass = vec_equal_mem<C>(r, nm1, L, L, sm.sc);
+        if (!pass && vec_equal_mem<C>(r, one, L, L, sm.sc)) break;     // reached 1: composite
     }
     if (threadIdx.x == 0 && !pass) D.alive[c] = 0;
 }
@@ -637,14 +704,15 @@ __global__ __launch_bounds__(THREADS) void debug_kernel(Dev D, Twiddles T, Const
     constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
     __shared__ Smem<LOGM> sm;
     const int c = blockIdx.x;
-    const int L = D.L;
+    const int L = D.Ls[c];
     uint32_t a[C], b[C];
     load_vec<C>(a, A + (size_t)c * stride2, L, 0);
     load_vec<C>(b, B + (size_t)c * stride2, L, 0);
     uint32_t *scr = D.scr + (size_t)c * 2 * M;
     if (op == 0) {
         to_fft<LOGM>(sm, T, K, b, L, scr);
-        tproduct<LOGM>(sm, T, K, a, 0, L, scr, a, M);
+        tproduct<LOGM>(sm, T, K, a, 0, L, scr, a, M,
+                       D.ysv + (G<LOGM>::NP == 1 ? (size_t)c * M : 0));
         store_vec<C>(a, out + (size_t)c * stride2, 2 * L, 0);
     } else if (op == 1) {
         to_fft<LOGM>(sm, T, K, b, L, scr);
@@ -698,7 +766,9 @@ Consts make_consts(int logm)
     return K;
 }
 
-constexpr int MIN_LOGM = 11, MAX_LOGM = 14;
+constexpr int MIN_LOGM = 11, MAX_LOGM = 15;
+int g_min_logm = MIN_LOGM;      // floor for new contexts (mr_set_min_log_len)
+constexpr bool G_NP1(int logm) { return logm > 14; }
 
 struct TwCache {
     uint32_t *tw[MAX_LOGM + 1] = {};
@@ -758,6 +828,7 @@ int dispatch_logm(int logm, F f)
     case 12: return f(std::integral_constant<int, 12>{});
     case 13: return f(std::integral_constant<int, 13>{});
     case 14: return f(std::integral_constant<int, 14>{});
+    case 15: return f(std::integral_constant<int, 15>{});
     default: return -1;
     }
 }
@@ -770,13 +841,27 @@ int ctx_alloc(mr_ctx *x, void **p, size_t bytes)
     return 0;
 }
 
-// Geometry and device arrays for numbers of at most `bits` bits.
-int ctx_alloc_all(mr_ctx *x, long bits)
+constexpr int TMAX = 24;       // sparse decimal terms per candidate
+
+// Limbs and log2 transform length for numbers of at most `bits` bits.
+void geometry(long bits, int min_logm, int *L_out, int *logm_out)
 {
     int L = (int)((bits + 23) / 24);
     if (L < 4) L = 4;
-    int logm = MIN_LOGM;
+    int logm = min_logm;
     while ((1 << logm) < 2 * L + 2) logm++;
+    *L_out = L;
+    *logm_out = logm;
+}
+
+// Upper bound on the bit length of a number of `digits` decimal digits.
+long digits_to_bits(int64_t digits) { return (long)((double)digits * 3.3219280948873623) + 2; }
+
+// Geometry and device arrays for numbers of at most `bits` bits.
+int ctx_alloc_all(mr_ctx *x, long bits, int min_logm)
+{
+    int L, logm;
+    geometry(bits, min_logm, &L, &logm);
     if (logm > MAX_LOGM || L > MAX_LIMBS) {
         fprintf(stderr, "%s: numbers of %ld bits are beyond this build's transform sizes\n", kWho, bits);
         return -1;
@@ -801,13 +886,20 @@ int ctx_alloc_all(mr_ctx *x, long bits)
     MR_ALLOC(nm1, uint32_t, n * x->stride);
     MR_ALLOC(res, uint32_t, n * x->stride);
     MR_ALLOC(scr, uint32_t, n * 2 * M);
+    MR_ALLOC(ysv, uint32_t, G_NP1(logm) ? n * M : 1);
     MR_ALLOC(nbits, int32_t, n);
     MR_ALLOC(sval, int32_t, n);
     MR_ALLOC(bitpos, int32_t, n);
     MR_ALLOC(alive, uint8_t, n);
     MR_ALLOC(pre, uint8_t, n);
     MR_ALLOC(err, uint32_t, 1);
+    MR_ALLOC(Ls, int32_t, n);
+    MR_ALLOC(base, uint32_t, n);
+    MR_ALLOC(bidx, int32_t, n);
+    MR_ALLOC(done, uint8_t, n);
 #undef MR_ALLOC
+    d.list = nullptr;
+    fill_i32_kernel<<<(x->n + 255) / 256, 256, 0, x->st>>>(d.Ls, x->n, L);
     uint32_t *tw = nullptr;
     if (get_twiddles(x->device, logm, x->st, &tw) != 0) return hip_fail(hipErrorUnknown, "twiddles");
     x->T.base = tw;
@@ -841,7 +933,7 @@ int ctx_finish(mr_ctx *x)
 
 int ctx_init(mr_ctx *x, const uint32_t *h_words)
 {
-    int rc = ctx_alloc_all(x, (long)x->n_words * 32);
+    int rc = ctx_alloc_all(x, (long)x->n_words * 32, g_min_logm);
     if (rc) return rc;
     uint32_t *dw = nullptr;
     const size_t n = (size_t)x->n;
@@ -854,6 +946,7 @@ int ctx_init(mr_ctx *x, const uint32_t *h_words)
 mr_ctx *ctx_new(int n_words, int n_batch, int device)
 {
     if (n_words < 1 || n_batch < 1) return nullptr;
+    if (device < 0 && hipGetDevice(&device) != hipSuccess) return nullptr;
     mr_ctx *x = new (std::nothrow) mr_ctx;
     if (!x) return nullptr;
     x->device = device;
@@ -959,9 +1052,9 @@ extern "C" int mr_run(mr_ctx_t *x, const uint32_t *bases, int n_bases, uint8_t *
         const uint32_t base = bases[bi];
         rc = dispatch_logm(x->logm, [&](auto lg) {
             constexpr int LG = decltype(lg)::value;
-            round_start_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, base);
+            round_start_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, base, 0, nullptr, 0);
             for (int done = 0; done < max_bits; done += CHUNK)
-                exp_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, x->T, x->K, base, CHUNK);
+                exp_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, x->T, x->K, CHUNK);
             if (residues_out) {
                 MR_CHECK(hipMemsetAsync(dlimbs, 0, sizeof(uint32_t) * (size_t)n * x->stride, x->st));
                 round_end_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, x->T, x->K, dlimbs, x->stride);
@@ -1136,7 +1229,6 @@ bool candidate_terms(const mr_candidate_t &cd, std::vector<std::pair<int64_t, in
 extern "C" mr_ctx_t *mr_ctx_create_sparse(const mr_candidate_t *cands, int n_batch, int device)
 {
     if (!cands || n_batch < 1) return nullptr;
-    const int TMAX = 24;
     std::vector<int64_t> terms((size_t)n_batch * TMAX * 2, 0);
     int64_t max_digits = 1;
     for (int i = 0; i < n_batch; i++) {
@@ -1154,11 +1246,10 @@ extern "C" mr_ctx_t *mr_ctx_create_sparse(const mr_candidate_t *cands, int n_bat
         }
         if (dg > max_digits) max_digits = dg;
     }
-    // bits of 10^D: floor(D log2 10) + 1, bounded from above exactly enough
-    const long bits = (long)((double)max_digits * 3.3219280948873623) + 2;
+    const long bits = digits_to_bits(max_digits);
     mr_ctx *x = ctx_new((int)((bits + 31) / 32), n_batch, device);
     if (!x) return nullptr;
-    int rc = ctx_alloc_all(x, (long)x->n_words * 32);
+    int rc = ctx_alloc_all(x, (long)x->n_words * 32, g_min_logm);
     int64_t *dterms = nullptr;
     if (rc == 0 && ctx_alloc(x, (void **)&dterms, sizeof(int64_t) * terms.size()) != 0) rc = -2;
     if (rc == 0 && hipMemcpyAsync(dterms, terms.data(), sizeof(int64_t) * terms.size(),
@@ -1166,7 +1257,7 @@ extern "C" mr_ctx_t *mr_ctx_create_sparse(const mr_candidate_t *cands, int n_bat
     if (rc == 0)
         rc = dispatch_logm(x->logm, [&](auto lg) {
             constexpr int LG = decltype(lg)::value;
-            build_kernel<LG><<<x->n, THREADS, 0, x->st>>>(x->d, x->T, x->K, dterms, TMAX);
+            build_kernel<LG><<<x->n, THREADS, 0, x->st>>>(x->d, x->T, x->K, dterms, TMAX, nullptr);
             return 0;
         });
     if (rc == 0) rc = ctx_finish(x);
@@ -1175,6 +1266,13 @@ extern "C" mr_ctx_t *mr_ctx_create_sparse(const mr_candidate_t *cands, int n_bat
         return nullptr;
     }
     return x;
+}
+
+extern "C" int mr_set_min_log_len(int log_len)
+{
+    if (log_len < MIN_LOGM || log_len > MAX_LOGM) return -1;
+    g_min_logm = log_len;
+    return 0;
 }
 
 extern "C" int mr_time_squarings(mr_ctx_t *x, uint32_t base, int n_bits, float *ms)
@@ -1188,12 +1286,12 @@ extern "C" int mr_time_squarings(mr_ctx_t *x, uint32_t base, int n_bits, float *
     init_alive_kernel<<<(n + 255) / 256, 256, 0, x->st>>>(x->d);
     int rc = dispatch_logm(x->logm, [&](auto lg) {
         constexpr int LG = decltype(lg)::value;
-        round_start_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, base);
+        round_start_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, base, 0, nullptr, 0);
         (void)hipEventRecord(e0, x->st);
         const int CHUNK = 256;
         for (int done = 0; done < n_bits; done += CHUNK) {
             const int steps = n_bits - done < CHUNK ? n_bits - done : CHUNK;
-            exp_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, x->T, x->K, base, steps);
+            exp_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, x->T, x->K, steps);
         }
         (void)hipEventRecord(e1, x->st);
         return 0;
@@ -1205,4 +1303,491 @@ extern "C" int mr_time_squarings(mr_ctx_t *x, uint32_t base, int n_bits, float *
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     return e == hipSuccess ? 0 : hip_fail(e, "mr_time_squarings");
+}
+
+// ------------------------------------------------------------ scheduler
+//
+// Slot state machine (host side; one entry per slot of a pool):
+//   FREE -> ADM (admin stream: build, setup, first round start)
+//   ADM  -> RUN (rounds to run) | FREE (decided without rounds)
+//   RUN  -> END (chunk finished the exponent: bitpos < s)
+//   END  -> ADM (admin stream: round end, next base or verdict)
+// A slot is touched by one stream at a time; chunks carry only RUN slots.
+
+namespace {
+
+enum : uint8_t { SL_FREE = 0, SL_ADM = 1, SL_RUN = 2, SL_END = 3 };
+
+double mono_now()
+{
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+struct Req {
+    mr_candidate_t cand;
+    int64_t tag;
+    double t_submit, t_admit;
+    int L, logm;
+    int rounds;
+    int64_t terms[TMAX * 2];
+};
+
+// Bytes of device memory per slot of a pool of transform length 2^logm
+// (the arrays of ctx_alloc_all).
+int64_t slot_bytes(int logm)
+{
+    const int64_t M = 1ll << logm;
+    return 4 * (4 * M + 6 * M + (G_NP1(logm) ? M : 0)) + 32;
+}
+
+struct Pool {
+    int logm = 0, n = 0;
+    mr_ctx *x = nullptr;            // x->st is the admin stream
+    hipStream_t st_run = nullptr;
+    hipEvent_t ev_run = nullptr, ev_adm = nullptr;
+    int32_t *d_list_run = nullptr, *d_list_adm = nullptr, *d_Ls = nullptr;
+    int64_t *d_terms = nullptr;
+    uint32_t *d_bases = nullptr;
+    // pinned host mirrors
+    int32_t *h_list_run = nullptr, *h_list_adm = nullptr, *h_Ls = nullptr;
+    int32_t *h_bitpos = nullptr, *h_sval = nullptr;
+    int64_t *h_terms = nullptr;
+    uint8_t *h_alive = nullptr, *h_done = nullptr;
+    uint32_t *h_err = nullptr;
+    std::vector<uint8_t> state;
+    std::vector<int> req;          // slot -> request index
+    std::vector<int> run_list, adm_list;
+    int adm_n_end = 0;
+    bool run_busy = false, adm_busy = false;
+    std::vector<int> pending;      // request indices, FIFO
+    size_t pend_head = 0;
+};
+
+} // namespace
+
+struct mr_sched {
+    int device = 0, n_bases = 0, max_active = 0, chunk = 256;
+    int64_t budget = 0, hbm = 0;
+    std::vector<uint32_t> bases;
+    std::vector<Req> reqs;
+    std::vector<mr_result_t> done;
+    size_t done_head = 0;
+    Pool *pools[MAX_LOGM + 1] = {};
+    int active = 0;
+    int64_t n_done = 0, chunks = 0, admins = 0, squarings = 0;
+    bool failed = false;
+};
+
+namespace {
+
+void pool_free(Pool *p)
+{
+    if (!p) return;
+    if (p->st_run) (void)hipStreamSynchronize(p->st_run);
+    if (p->x && p->x->st) (void)hipStreamSynchronize(p->x->st);
+    if (p->ev_run) (void)hipEventDestroy(p->ev_run);
+    if (p->ev_adm) (void)hipEventDestroy(p->ev_adm);
+    if (p->st_run) (void)hipStreamDestroy(p->st_run);
+    void *hp[] = {p->h_list_run, p->h_list_adm, p->h_Ls, p->h_bitpos, p->h_sval,
+                  p->h_terms, p->h_alive, p->h_done, p->h_err};
+    for (void *q : hp) if (q) (void)hipHostFree(q);
+    if (p->x) mr_ctx_destroy(p->x);       // frees the device arrays it owns
+    delete p;
+}
+
+int pool_new(mr_sched *S, int logm, Pool **out)
+{
+    const int64_t per = slot_bytes(logm);
+    int64_t room = (S->budget - S->hbm) / per;
+    int n = S->max_active;
+    if (room < n) n = (int)room;
+    if (n < 1) {
+        fprintf(stderr, "%s: HBM budget exhausted (%lld bytes in use)\n", kWho, (long long)S->hbm);
+        return -1;
+    }
+    Pool *p = new (std::nothrow) Pool;
+    if (!p) return -2;
+    p->logm = logm;
+    p->n = n;
+    p->x = ctx_new(1, n, S->device);
+    int rc = p->x ? 0 : -2;
+    // numbers of up to (2^logm - 2) / 2 limbs: the pool's transform length
+    if (rc == 0) rc = ctx_alloc_all(p->x, 24l * (((1l << logm) - 2) / 2), logm);
+    if (rc == 0 && p->x->logm != logm) rc = -1;
+    auto dev = [&](void **q, size_t b) { if (rc == 0 && ctx_alloc(p->x, q, b) != 0) rc = -2; };
+    dev((void **)&p->d_list_run, 4 * (size_t)n);
+    dev((void **)&p->d_list_adm, 4 * (size_t)n);
+    dev((void **)&p->d_Ls, 4 * (size_t)n);
+    dev((void **)&p->d_terms, sizeof(int64_t) * TMAX * 2 * (size_t)n);
+    dev((void **)&p->d_bases, 4 * (size_t)S->n_bases);
+    auto host = [&](void **q, size_t b) {
+        if (rc == 0 && hipHostMalloc(q, b ? b : 4, hipHostMallocDefault) != hipSuccess) rc = -2;
+    };
+    host((void **)&p->h_list_run, 4 * (size_t)n);
+    host((void **)&p->h_list_adm, 4 * (size_t)n);
+    host((void **)&p->h_Ls, 4 * (size_t)n);
+    host((void **)&p->h_bitpos, 4 * (size_t)n);
+    host((void **)&p->h_sval, 4 * (size_t)n);
+    host((void **)&p->h_terms, sizeof(int64_t) * TMAX * 2 * (size_t)n);
+    host((void **)&p->h_alive, (size_t)n);
+    host((void **)&p->h_done, (size_t)n);
+    host((void **)&p->h_err, 4);
+    if (rc == 0 && (hipStreamCreateWithFlags(&p->st_run, hipStreamNonBlocking) != hipSuccess ||
+                    hipEventCreateWithFlags(&p->ev_run, hipEventDisableTiming) != hipSuccess ||
+                    hipEventCreateWithFlags(&p->ev_adm, hipEventDisableTiming) != hipSuccess))
+        rc = -2;
+    if (rc == 0 && hipMemcpyAsync(p->d_bases, S->bases.data(), 4 * (size_t)S->n_bases,
+                                  hipMemcpyHostToDevice, p->x->st) != hipSuccess)
+        rc = -2;
+    if (rc == 0 && hipStreamSynchronize(p->x->st) != hipSuccess) rc = -2;
+    if (rc != 0) {
+        pool_free(p);
+        return rc;
+    }
+    p->state.assign(n, SL_FREE);
+    p->req.assign(n, -1);
+    S->hbm += per * n;
+    *out = p;
+    return 0;
+}
+
+void finish(mr_sched *S, int r, int verdict)
+{
+    Req &q = S->reqs[r];
+    mr_result_t o;
+    o.tag = q.tag;
+    o.verdict = verdict;
+    o.rounds = q.rounds;
+    o.t_submit = q.t_submit;
+    o.t_admit = q.t_admit;
+    o.t_done = mono_now();
+    S->done.push_back(o);
+    S->n_done++;
+}
+
+// Launch one squaring chunk over the pool's RUN slots.
+int launch_chunk(mr_sched *S, Pool *p)
+{
+    p->run_list.clear();
+    for (int c = 0; c < p->n; c++)
+        if (p->state[c] == SL_RUN) p->run_list.push_back(c);
+    if (p->run_list.empty()) return 0;
+    const int nr = (int)p->run_list.size();
+    memcpy(p->h_list_run, p->run_list.data(), 4 * (size_t)nr);
+    MR_CHECK(hipMemcpyAsync(p->d_list_run, p->h_list_run, 4 * (size_t)nr, hipMemcpyHostToDevice, p->st_run));
+    Dev Dl = p->x->d;
+    Dl.list = p->d_list_run;
+    const int chunk = S->chunk;
+    int rc = dispatch_logm(p->logm, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        exp_kernel<LG><<<nr, THREADS, 0, p->st_run>>>(Dl, p->x->T, p->x->K, chunk);
+        return 0;
+    });
+    if (rc) return rc;
+    MR_CHECK(hipGetLastError());
+    MR_CHECK(hipMemcpyAsync(p->h_bitpos, p->x->d.bitpos, 4 * (size_t)p->n, hipMemcpyDeviceToHost, p->st_run));
+    MR_CHECK(hipEventRecord(p->ev_run, p->st_run));
+    p->run_busy = true;
+    S->chunks++;
+    S->squarings += (int64_t)nr * chunk;
+    return 0;
+}
+
+void chunk_done(Pool *p)
+{
+    for (int c : p->run_list)
+        if (p->h_bitpos[c] < p->h_sval[c]) p->state[c] = SL_END;
+    p->run_busy = false;
+}
+
+// Admin batch: end the rounds of END slots, admit pending requests into
+// FREE slots (within the scheduler's active cap).
+int launch_admin(mr_sched *S, Pool *p)
+{
+    p->adm_list.clear();
+    for (int c = 0; c < p->n; c++)
+        if (p->state[c] == SL_END) p->adm_list.push_back(c);
+    const int n_end = (int)p->adm_list.size();
+    for (int c = 0; c < p->n && p->pend_head < p->pending.size() && S->active < S->max_active; c++) {
+        if (p->state[c] != SL_FREE) continue;
+        const int r = p->pending[p->pend_head++];
+        Req &q = S->reqs[r];
+        const int i = (int)p->adm_list.size() - n_end;
+        memcpy(p->h_terms + (size_t)i * TMAX * 2, q.terms, sizeof(q.terms));
+        p->h_Ls[i] = q.L;
+        q.t_admit = mono_now();
+        p->req[c] = r;
+        p->adm_list.push_back(c);
+        S->active++;
+    }
+    const int n_all = (int)p->adm_list.size(), n_new = n_all - n_end;
+    if (n_all == 0) return 0;
+    for (int c : p->adm_list) p->state[c] = SL_ADM;
+    p->adm_n_end = n_end;
+    hipStream_t st = p->x->st;
+    memcpy(p->h_list_adm, p->adm_list.data(), 4 * (size_t)n_all);
+    MR_CHECK(hipMemcpyAsync(p->d_list_adm, p->h_list_adm, 4 * (size_t)n_all, hipMemcpyHostToDevice, st));
+    if (n_new) {
+        MR_CHECK(hipMemcpyAsync(p->d_terms, p->h_terms, sizeof(int64_t) * TMAX * 2 * (size_t)n_new,
+                                hipMemcpyHostToDevice, st));
+        MR_CHECK(hipMemcpyAsync(p->d_Ls, p->h_Ls, 4 * (size_t)n_new, hipMemcpyHostToDevice, st));
+    }
+    Dev De = p->x->d, Dn = p->x->d;
+    De.list = p->d_list_adm;
+    Dn.list = p->d_list_adm + n_end;
+    const Twiddles &T = p->x->T;
+    const Consts &K = p->x->K;
+    const uint32_t *bases = p->d_bases;
+    const int nb = S->n_bases;
+    int rc = dispatch_logm(p->logm, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        if (n_end) {
+            round_end_kernel<LG><<<n_end, THREADS, 0, st>>>(De, T, K, nullptr, 0);
+            round_start_kernel<LG><<<n_end, THREADS, 0, st>>>(De, 0u, 2, bases, nb);
+        }
+        if (n_new) {
+            build_kernel<LG><<<n_new, THREADS, 0, st>>>(Dn, T, K, p->d_terms, TMAX, p->d_Ls);
+            setup_kernel<LG><<<n_new, THREADS, 0, st>>>(Dn, T, K);
+            round_start_kernel<LG><<<n_new, THREADS, 0, st>>>(Dn, 0u, 1, bases, nb);
+        }
+        return 0;
+    });
+    if (rc) return rc;
+    MR_CHECK(hipGetLastError());
+    const size_t n = (size_t)p->n;
+    MR_CHECK(hipMemcpyAsync(p->h_alive, p->x->d.alive, n, hipMemcpyDeviceToHost, st));
+    MR_CHECK(hipMemcpyAsync(p->h_done, p->x->d.done, n, hipMemcpyDeviceToHost, st));
+    MR_CHECK(hipMemcpyAsync(p->h_sval, p->x->d.sval, 4 * n, hipMemcpyDeviceToHost, st));
+    MR_CHECK(hipMemcpyAsync(p->h_err, p->x->d.err, 4, hipMemcpyDeviceToHost, st));
+    MR_CHECK(hipEventRecord(p->ev_adm, st));
+    p->adm_busy = true;
+    S->admins++;
+    return 0;
+}
+
+int admin_done(mr_sched *S, Pool *p)
+{
+    p->adm_busy = false;
+    if (*p->h_err) {
+        fprintf(stderr, "%s: a candidate does not fit its slot\n", kWho);
+        return -2;
+    }
+    for (int i = 0; i < (int)p->adm_list.size(); i++) {
+        const int c = p->adm_list[i];
+        const int r = p->req[c];
+        if (i < p->adm_n_end) S->reqs[r].rounds++;     // the round that just ended
+        if (p->h_done[c]) {
+            finish(S, r, p->h_alive[c] ? 1 : 0);
+            p->state[c] = SL_FREE;
+            p->req[c] = -1;
+            S->active--;
+        } else {
+            p->state[c] = SL_RUN;
+        }
+    }
+    return 0;
+}
+
+int sched_pending(const mr_sched *S)
+{
+    return (int)((int64_t)S->reqs.size() - S->n_done);
+}
+
+} // namespace
+
+extern "C" mr_sched_t *mr_sched_create(const uint32_t *bases, int n_bases, int device,
+                                       int max_active, int chunk, int64_t hbm_budget)
+{
+    if (!bases || n_bases < 1 || chunk < 0 || max_active < 0 || hbm_budget < 0) return nullptr;
+    for (int i = 0; i < n_bases; i++)
+        if (bases[i] < 2u || bases[i] >= (1u << 24)) return nullptr;
+    if (device < 0 && hipGetDevice(&device) != hipSuccess) return nullptr;
+    if (hipSetDevice(device) != hipSuccess) return nullptr;
+    if (max_active == 0) {
+        int cus = 0;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+            return nullptr;
+        max_active = cus;
+    }
+    mr_sched *S = new (std::nothrow) mr_sched;
+    if (!S) return nullptr;
+    S->device = device;
+    S->n_bases = n_bases;
+    S->bases.assign(bases, bases + n_bases);
+    S->max_active = max_active;
+    S->chunk = chunk ? chunk : 256;
+    S->budget = hbm_budget ? hbm_budget : (64ll << 30);
+    return S;
+}
+
+extern "C" int mr_sched_submit(mr_sched_t *S, const mr_candidate_t *cands, const int64_t *tags, int n)
+{
+    if (!S || (!cands && n) || n < 0) return -1;
+    std::vector<Req> add((size_t)n);
+    std::vector<std::pair<int64_t, int64_t>> t;
+    for (int i = 0; i < n; i++) {
+        Req &q = add[i];
+        int64_t dg = 0;
+        if (!candidate_terms(cands[i], t, &dg) || (int)t.size() > TMAX) {
+            fprintf(stderr, "%s: invalid candidate (a=%lld b=%lld c=%lld k=%u j=%u which=%d)\n", kWho,
+                    (long long)cands[i].a, (long long)cands[i].b, (long long)cands[i].c, cands[i].k,
+                    cands[i].j, cands[i].which);
+            return -1;
+        }
+        geometry(digits_to_bits(dg), MIN_LOGM, &q.L, &q.logm);
+        if (q.logm > MAX_LOGM || q.L > MAX_LIMBS) {
+            fprintf(stderr, "%s: candidate of %lld digits is beyond this build's transform sizes\n",
+                    kWho, (long long)dg);
+            return -1;
+        }
+        memset(q.terms, 0, sizeof(q.terms));
+        for (size_t k = 0; k < t.size(); k++) {
+            q.terms[2 * k] = t[k].first;
+            q.terms[2 * k + 1] = t[k].second;
+        }
+        q.cand = cands[i];
+        q.tag = tags ? tags[i] : (int64_t)(S->reqs.size() + i);
+        q.rounds = 0;
+        q.t_admit = 0.0;
+    }
+    const double now = mono_now();
+    if (hipSetDevice(S->device) != hipSuccess) return -2;
+    for (int i = 0; i < n; i++) {
+        Req &q = add[i];
+        Pool *&p = S->pools[q.logm];
+        if (!p) {
+            int rc = pool_new(S, q.logm, &p);
+            if (rc) { p = nullptr; return rc; }
+        }
+        q.t_submit = now;
+        p->pending.push_back((int)S->reqs.size());
+        S->reqs.push_back(q);
+    }
+    return 0;
+}
+
+extern "C" int mr_sched_pump(mr_sched_t *S, int block)
+{
+    if (!S) return -1;
+    if (S->failed) return -2;
+    if (hipSetDevice(S->device) != hipSuccess) return -2;
+    auto fail = [&](int rc) { S->failed = true; return rc < 0 ? rc : -2; };
+    for (;;) {
+        // 1. collect finished work
+        for (Pool *p : S->pools) {
+            if (!p) continue;
+            if (p->run_busy) {
+                hipError_t e = hipEventQuery(p->ev_run);
+                if (e == hipSuccess) chunk_done(p);
+                else if (e != hipErrorNotReady) return fail(hip_fail(e, "chunk"));
+            }
+            if (p->adm_busy) {
+                hipError_t e = hipEventQuery(p->ev_adm);
+                if (e == hipSuccess) { int rc = admin_done(S, p); if (rc) return fail(rc); }
+                else if (e != hipErrorNotReady) return fail(hip_fail(e, "admission"));
+            }
+        }
+        // 2. launch: admissions oldest-first over the pools, then chunks
+        Pool *order[MAX_LOGM + 1];
+        int np = 0;
+        for (Pool *p : S->pools) if (p) order[np++] = p;
+        for (int a = 1; a < np; a++)       // by submit time of the head request
+            for (int b = a; b > 0; b--) {
+                auto head = [&](Pool *p) {
+                    return p->pend_head < p->pending.size() ? S->reqs[p->pending[p->pend_head]].t_submit
+                                                            : 1e300;
+                };
+                if (head(order[b]) < head(order[b - 1])) std::swap(order[b], order[b - 1]);
+                else break;
+            }
+        for (int i = 0; i < np; i++) {
+            Pool *p = order[i];
+            if (!p->adm_busy) { int rc = launch_admin(S, p); if (rc) return fail(rc); }
+        }
+        for (int i = 0; i < np; i++) {
+            Pool *p = order[i];
+            if (!p->run_busy) { int rc = launch_chunk(S, p); if (rc) return fail(rc); }
+        }
+        const int left = sched_pending(S);
+        bool busy = false;
+        for (Pool *p : S->pools) if (p && (p->run_busy || p->adm_busy)) busy = true;
+        if (!busy && left > 0) {       // every undecided candidate is queued or in flight
+            fprintf(stderr, "%s: scheduler stalled with %d candidates left\n", kWho, left);
+            return fail(-2);
+        }
+        if (!block || !busy) return left;
+        // 3. wait for the first chunk in flight (else an admission batch)
+        hipEvent_t ev = nullptr;
+        for (Pool *p : S->pools) if (p && p->run_busy) { ev = p->ev_run; break; }
+        if (!ev)
+            for (Pool *p : S->pools) if (p && p->adm_busy) { ev = p->ev_adm; break; }
+        hipError_t e = hipEventSynchronize(ev);
+        if (e != hipSuccess) return fail(hip_fail(e, "wait"));
+        block = 0;      // one wait per call; loop once more to act on it
+    }
+}
+
+extern "C" int mr_sched_results(mr_sched_t *S, mr_result_t *out, int cap)
+{
+    if (!S || (!out && cap) || cap < 0) return -1;
+    int k = 0;
+    while (k < cap && S->done_head < S->done.size()) out[k++] = S->done[S->done_head++];
+    if (S->done_head == S->done.size()) { S->done.clear(); S->done_head = 0; }
+    return k;
+}
+
+extern "C" int mr_sched_stats(const mr_sched_t *S, mr_sched_stats_t *o)
+{
+    if (!S || !o) return -1;
+    o->submitted = (int64_t)S->reqs.size();
+    o->completed = S->n_done;
+    o->active = S->active;
+    o->pending = o->submitted - o->completed - o->active;
+    o->chunks = S->chunks;
+    o->admin_batches = S->admins;
+    o->squarings = S->squarings;
+    o->hbm_bytes = S->hbm;
+    o->max_active = S->max_active;
+    int np = 0;
+    for (Pool *p : S->pools) np += p != nullptr;
+    o->n_pools = np;
+    return 0;
+}
+
+extern "C" void mr_sched_destroy(mr_sched_t *S)
+{
+    if (!S) return;
+    (void)hipSetDevice(S->device);
+    for (Pool *&p : S->pools) { pool_free(p); p = nullptr; }
+    delete S;
+}
+
+extern "C" int mr_test_batch(const mr_candidate_t *cands, int n, const uint32_t *bases, int n_bases,
+                             int device, uint8_t *verdict, int32_t *rounds)
+{
+    if (!cands || n < 1 || !verdict) return -1;
+    mr_sched_t *S = mr_sched_create(bases, n_bases, device, 0, 0, 0);
+    if (!S) return -1;
+    std::vector<int64_t> tags((size_t)n);
+    for (int i = 0; i < n; i++) tags[i] = i;
+    int rc = mr_sched_submit(S, cands, tags.data(), n);
+    while (rc == 0) {
+        const int left = mr_sched_pump(S, 1);
+        if (left < 0) { rc = left; break; }
+        if (left == 0) break;
+    }
+    if (rc == 0) {
+        std::vector<mr_result_t> res((size_t)n);
+        const int k = mr_sched_results(S, res.data(), n);
+        for (int i = 0; i < k; i++) {
+            const int64_t t = res[i].tag;
+            verdict[t] = (uint8_t)res[i].verdict;
+            if (rounds) rounds[t] = res[i].rounds;
+        }
+        if (k != n) rc = -2;
+    }
+    mr_sched_destroy(S);
+    return rc;
 }

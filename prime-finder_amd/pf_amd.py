@@ -212,6 +212,17 @@ def _mr_sigs(L):
         "mr_debug_montmul": (i32, [vp, u32p, u32p, u32p]),
         "mr_debug_mulsmall": (i32, [vp, u32p, u32, u32p]),
         "mr_time_squarings": (i32, [vp, u32, i32, C.POINTER(C.c_float)]),
+        "mr_set_min_log_len": (i32, [i32]),
+        "miller_rabin_test": (i32, [C.c_longlong, C.c_longlong, C.c_longlong, i32, i32,
+                                    C.POINTER(MrTiming)]),
+        "mr_sched_create": (vp, [C.POINTER(u32), i32, i32, i32, i32, C.c_int64]),
+        "mr_sched_submit": (i32, [vp, C.POINTER(MrCandidate), C.POINTER(C.c_int64), i32]),
+        "mr_sched_pump": (i32, [vp, i32]),
+        "mr_sched_results": (i32, [vp, C.POINTER(MrResult), i32]),
+        "mr_sched_stats": (i32, [vp, C.POINTER(MrSchedStats)]),
+        "mr_sched_destroy": (None, [vp]),
+        "mr_test_batch": (i32, [C.POINTER(MrCandidate), i32, C.POINTER(u32), i32, i32,
+                                C.POINTER(C.c_uint8), C.POINTER(C.c_int32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -229,16 +240,133 @@ def words_to_int(w: np.ndarray) -> int:
     return int.from_bytes(np.ascontiguousarray(w, dtype=np.uint32).tobytes(), "little")
 
 
+class MrResult(C.Structure):
+    """mr_result_t (include/mr_gpu.h)."""
+    _fields_ = [("tag", C.c_int64), ("verdict", C.c_int32), ("rounds", C.c_int32),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+class MrSchedStats(C.Structure):
+    """mr_sched_stats_t (include/mr_gpu.h)."""
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("pending", C.c_int64),
+                ("active", C.c_int64), ("chunks", C.c_int64), ("admin_batches", C.c_int64),
+                ("squarings", C.c_int64), ("hbm_bytes", C.c_int64),
+                ("max_active", C.c_int32), ("n_pools", C.c_int32)]
+
+
+def _cands(cands):
+    arr = (MrCandidate * len(cands))()
+    for i, cd in enumerate(cands):
+        a, b, c, k, j, w = cd
+        arr[i] = MrCandidate(a, b, c, k, j, w)
+    return arr
+
+
+class MrScheduler:
+    """Continuous batching over mixed-size candidates (mr_sched_* C ABI)."""
+
+    def __init__(self, bases, device: int = 0, max_active: int = 0, chunk: int = 0,
+                 hbm_budget: int = 0):
+        L = mr_lib()
+        self.L = L
+        b = (C.c_uint32 * len(bases))(*bases)
+        self.ptr = L.mr_sched_create(b, len(bases), device, max_active, chunk, hbm_budget)
+        if not self.ptr:
+            raise RuntimeError("mr_sched_create failed")
+
+    def submit(self, cands, tags=None):
+        arr = _cands(cands)
+        t = None
+        if tags is not None:
+            t = (C.c_int64 * len(cands))(*tags)
+        if self.L.mr_sched_submit(self.ptr, arr, t, len(cands)) != 0:
+            raise ValueError("mr_sched_submit rejected the candidates")
+
+    def pump(self, block: bool = True) -> int:
+        left = self.L.mr_sched_pump(self.ptr, 1 if block else 0)
+        if left < 0:
+            raise RuntimeError(f"mr_sched_pump failed ({left})")
+        return left
+
+    def results(self, cap: int = 4096):
+        out = (MrResult * cap)()
+        k = self.L.mr_sched_results(self.ptr, out, cap)
+        if k < 0:
+            raise RuntimeError("mr_sched_results failed")
+        return [out[i] for i in range(k)]
+
+    def stats(self) -> MrSchedStats:
+        st = MrSchedStats()
+        if self.L.mr_sched_stats(self.ptr, C.byref(st)) != 0:
+            raise RuntimeError("mr_sched_stats failed")
+        return st
+
+    def run_all(self):
+        res = []
+        while True:
+            left = self.pump(True)
+            res += self.results()
+            if left == 0:
+                return res + self.results()
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self.L.mr_sched_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mr_test_batch(cands, bases, device: int = 0):
+    """verdicts (list of 0/1) and rounds run per candidate."""
+    L = mr_lib()
+    n = len(cands)
+    v = (C.c_uint8 * n)()
+    r = (C.c_int32 * n)()
+    b = (C.c_uint32 * len(bases))(*bases)
+    rc = L.mr_test_batch(_cands(cands), n, b, len(bases), device, v, r)
+    if rc != 0:
+        raise RuntimeError(f"mr_test_batch failed ({rc})")
+    return list(v), list(r)
+
+
+class MrTiming(C.Structure):
+    """mr_timing_t (include/miller_rabin.h; reference miller_rabin.h:3-8)."""
+    _fields_ = [("build_n", C.c_double), ("mr_n", C.c_double),
+                ("build_revn", C.c_double), ("mr_revn", C.c_double)]
+
+
+def miller_rabin_test(a: int, b: int, c: int, k: int, j: int):
+    """The reference's Phase-2 call (miller_rabin.h:24-25) on the GPU:
+    returns (code, MrTiming) with code 0 / 1 / 2 as the reference, -1 on
+    a candidate it cannot form or a HIP error."""
+    t = MrTiming()
+    code = mr_lib().miller_rabin_test(a, b, c, k, j, C.byref(t))
+    return code, t
+
+
+def mr_lib():
+    L = lib()
+    if not hasattr(L, "_mr_ready"):
+        _mr_sigs(L)
+        L._mr_ready = True
+    return L
+
+
 class MrContext:
-    """A batch of moduli resident on the GPU (include/mr_gpu.h)."""
+    """A batch of moduli resident on the GPU (include/mr_gpu.h).
+    min_log_len forces a transform of at least 2^min_log_len points."""
 
     def __init__(self, moduli=None, n_words: int | None = None, device: int = 0,
-                 candidates=None):
-        L = lib()
-        if not hasattr(L, "_mr_ready"):
-            _mr_sigs(L)
-            L._mr_ready = True
+                 candidates=None, min_log_len: int = 11):
+        L = mr_lib()
         self.L = L
+        if L.mr_set_min_log_len(min_log_len) != 0:
+            raise ValueError("min_log_len out of range")
         if candidates is not None:
             arr = (MrCandidate * len(candidates))(*[MrCandidate(*c) for c in candidates])
             self.ptr = L.mr_ctx_create_sparse(arr, len(candidates), device)
@@ -247,6 +375,7 @@ class MrContext:
             nw = n_words or max(1, max((m.bit_length() + 31) // 32 for m in moduli))
             w = np.concatenate([int_to_words(m, nw) for m in moduli])
             self.ptr = L.mr_ctx_create(w.ctypes.data_as(u32p), nw, len(moduli), device)
+        L.mr_set_min_log_len(11)
         if not self.ptr:
             raise RuntimeError("mr_ctx_create failed")
         nw_, limbs, logm = C.c_int(), C.c_int(), C.c_int()

@@ -40,6 +40,7 @@ def test_library_exports_every_declared_function(pf):
     syms = exported_symbols(pf.LIB_PATH)
     decl = declared_functions()
     assert "trial_div_gpu" in decl and "cuda_set_device" in decl
+    assert "miller_rabin_test" in decl and "mr_run" in decl
     missing = []
     for name, hdr in decl.items():
         if name == "trial_div_launch":
@@ -54,6 +55,7 @@ def test_library_exports_every_declared_function(pf):
 def test_reference_abi_struct_layouts(pf):
     assert C.sizeof(pf.PrimeList) == 16          # {uint32_t *, uint32_t} on LP64
     assert C.sizeof(pf.GpuTiming) == 24
+    assert C.sizeof(pf.MrTiming) == 32          # four doubles, miller_rabin.h:3-8
 
 
 def test_prime_sieve_matches_numpy(pf):
@@ -100,3 +102,13 @@ def test_prime_list_load_reference_format(pf, tmp_path):
     # truncated file -> error
     f.write_bytes(f.read_bytes()[:-2])
     assert L.prime_list_load(C.byref(pl), str(f).encode()) == -1
+
+
+def test_scheduler_rejects_bad_arguments_without_gpu(pf):
+    L = pf.mr_lib()
+    import ctypes as C
+    bad = (C.c_uint32 * 2)(2, 1)                 # base 1 is not allowed
+    assert not L.mr_sched_create(bad, 2, 0, 0, 0, 0)
+    assert not L.mr_sched_create(None, 0, 0, 0, 0, 0)
+    assert L.mr_sched_pump(None, 0) == -1
+    assert L.mr_sched_submit(None, None, None, 0) == -1

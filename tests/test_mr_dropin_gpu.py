@@ -1,0 +1,69 @@
+"""miller_rabin_test (include/miller_rabin.h) on the GPU against the
+reference's own Phase-2 checks (tests/test_phase2.c:46-71: N prime by trial
+division) and against GMP's mpz_probab_prime_p on N and the reference rev
+formula (the oracle's restatement of miller_rabin.c:32-118)."""
+import pytest
+
+from _support import gmp_build, gmp_probab_prime
+
+pytestmark = pytest.mark.gpu
+
+
+def is_prime_td(n: int) -> bool:
+    if n < 2:
+        return False
+    d = 2
+    while d * d <= n:
+        if n % d == 0:
+            return False
+        d += 1
+    return True
+
+
+def reference_code(a, b, c, k, j) -> int:
+    """miller_rabin.c:76-118 with GMP: 0 / 1 / 2."""
+    if gmp_probab_prime(gmp_build(a, b, c, k, j, 0)) == 0:
+        return 0
+    return 2 if gmp_probab_prime(gmp_build(a, b, c, k, j, 2)) > 0 else 1
+
+
+def test_phase2_small_vs_trial_division(gpu):
+    # test_phase2.c:51-69: N = 10^4 - 100k - 10j - 1, k, j in 1..5
+    for k in range(1, 6):
+        for j in range(1, 6):
+            n = 10000 - 100 * k - 10 * j - 1
+            code, _ = gpu.miller_rabin_test(4, 2, 1, k, j)
+            assert code in (0, 1, 2)
+            assert (code >= 1) == is_prime_td(n), (k, j, n, code)
+
+
+def test_codes_match_reference_semantics(gpu):
+    # test_phase2.c:37-44 grid (a=6, b=3, c=2) and (5, 2, 1, 1, 1), (4, 2, 1, 3, 5)
+    cases = [(6, 3, 2, k, j) for k in range(1, 11) for j in range(1, 11)]
+    cases += [(5, 2, 1, 1, 1), (4, 2, 1, 3, 5), (9, 4, 1, 7, 3), (12, 7, 3, 11, 9)]
+    seen = set()
+    for cs in cases:
+        code, t = gpu.miller_rabin_test(*cs)
+        assert code == reference_code(*cs), cs
+        seen.add(code)
+        if code == 0:
+            assert t.build_revn == 0.0 and t.mr_revn == 0.0
+    assert seen == {0, 1, 2}
+
+
+def test_codes_at_1000_digits(gpu):
+    # (k, j) of the triple (1000, 500, 100) where GMP finds N prime (code 1),
+    # mixed with composites
+    cases = [(32, 5), (47, 57), (54, 26), (69, 14), (1, 7), (2, 8), (3, 9)]
+    codes = []
+    for k, j in cases:
+        cs = (1000, 500, 100, k, j)
+        code, _ = gpu.miller_rabin_test(*cs)
+        assert code == reference_code(*cs), cs
+        codes.append(code)
+    assert codes == [1, 1, 1, 1, 0, 0, 0]
+
+
+def test_invalid_candidates_fail_loudly(gpu):
+    assert gpu.miller_rabin_test(4, 2, 1, 0, 1)[0] == -1        # k < 1
+    assert gpu.miller_rabin_test(3, 5, 1, 1, 1)[0] == -1        # N < 0

@@ -218,3 +218,62 @@ def test_even_and_tiny_moduli_are_decided(gpu):
     alive, res = ctx.run([2, 3])
     assert alive == [True, True, False, False, False, True, True]
     ctx.close()
+
+
+@pytest.mark.parametrize("logm", [12, 13, 15])
+def test_every_transform_length(gpu, logm):
+    """The same moduli through forced transform lengths 2^12 .. 2^15: exact
+    products, Montgomery products and full rounds against Python/GMP."""
+    rng = random.Random(logm)
+    mods = [rand_odd(rng, 1000), rand_odd(rng, 2500), 2 ** 1279 - 1, 3215031751]
+    ctx = gpu.MrContext(mods, min_log_len=logm)
+    assert ctx.log_len == logm
+    a = [rng.randrange(n) for n in mods]
+    b = [rng.randrange(n) for n in mods]
+    assert ctx.debug_product(a, b) == [x * y for x, y in zip(a, b)]
+    R = ctx.R
+    assert ctx.debug_montmul(a, b) == [x * y * pow(R, -1, n) % n for x, y, n in zip(a, b, mods)]
+    alive, res = ctx.run([2, 3, 5, 7])
+    for i, n in enumerate(mods):
+        exp_alive = True
+        for bi, base in enumerate((2, 3, 5, 7)):
+            ok, r, _ = sprp_python(n, base)
+            if exp_alive:
+                assert res[i][bi] == r
+            exp_alive = exp_alive and ok
+        assert alive[i] == exp_alive
+    ctx.close()
+
+
+def test_100k_digit_arithmetic(gpu):
+    """At 100 000 digits (2^15 transforms): device-built candidates equal
+    GMP, products and Montgomery products are exact."""
+    from _support import gmp_build
+    rng = random.Random(100000)
+    cands = [(100000, 61234, 789, 4095, 17, 0), (100000, 61234, 789, 4095, 17, 1),
+             (99990, 12345, 6789, 123, 4096, 0)]
+    ctx = gpu.MrContext(candidates=cands)
+    say("100k digits: L", ctx.limbs, "logM", ctx.log_len)
+    assert ctx.log_len == 15
+    mods = ctx.moduli()
+    for cd, n in zip(cands, mods):
+        assert n == gmp_build(*cd)
+    a = [rng.randrange(n) for n in mods]
+    b = [rng.randrange(n) for n in mods]
+    a[0] = mods[0] - 1
+    R = ctx.R
+    assert ctx.debug_product(a, b) == [x * y for x, y in zip(a, b)]
+    assert ctx.debug_montmul(a, b) == [x * y * pow(R, -1, n) % n for x, y, n in zip(a, b, mods)]
+    ctx.close()
+
+
+def test_mersenne_216091_closed_form(gpu):
+    """M216091 (65 050 digits, 2^15 transforms): M - 1 = 2 d with d odd and
+    3 a quadratic non-residue, so the base-3 residue 3^d is M - 1."""
+    m = 2 ** 216091 - 1
+    ctx = gpu.MrContext([m])
+    say("M216091: L", ctx.limbs, "logM", ctx.log_len, "- one base-3 round")
+    assert ctx.log_len == 15
+    alive, r3 = ctx.run([3])
+    assert r3[0][0] == m - 1 and alive[0]
+    ctx.close()

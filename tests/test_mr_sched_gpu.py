@@ -1,0 +1,116 @@
+"""Continuous batching (mr_sched_* / mr_test_batch): mixed sizes across
+transform buckets, slot reuse, early retirement and serving-style
+submission, checked against GMP strong-probable-prime rounds base by base
+(the oracle's mpz_powm harness)."""
+import random
+import sys
+import time
+
+import pytest
+
+from _support import gmp_build, gmp_sprp
+
+pytestmark = pytest.mark.gpu
+
+BASES = [2, 3, 5, 7]
+
+
+def say(*a):
+    print("[mr-sched]", *a, flush=True)
+
+
+def expected(cd, bases=BASES):
+    """(verdict, rounds) of per-base Miller-Rabin with retirement."""
+    n = gmp_build(*cd)
+    rounds = 0
+    for b in bases:
+        rounds += 1
+        ok, _, _ = gmp_sprp(n, b)
+        if not ok:
+            return 0, rounds
+    return 1, rounds
+
+
+def mixed_candidates(rng, digits, count, which=(0, 1, 2)):
+    out = []
+    for _ in range(count):
+        a = digits
+        b = rng.randrange(a // 3, a - 10)
+        c = rng.randrange(1, b // 2)
+        out.append((a, b, c, rng.randrange(1, 4097), rng.randrange(1, 4097), rng.choice(which)))
+    return out
+
+
+# (1000, 500, 100, k, j, 0) is prime for these (k, j) (GMP BPSW)
+PRIMES_1000 = [(1000, 500, 100, 32, 5, 0), (1000, 500, 100, 47, 57, 0), (1000, 500, 100, 54, 26, 0)]
+
+
+def test_mixed_sizes_small_pool(gpu):
+    """Four slots, short chunks: many round ends and refills across the
+    2^11, 2^12 and 2^13 buckets."""
+    rng = random.Random(7)
+    cands = (mixed_candidates(rng, 40, 6) + mixed_candidates(rng, 300, 6)
+             + mixed_candidates(rng, 1000, 4) + PRIMES_1000
+             + mixed_candidates(rng, 9000, 2) + mixed_candidates(rng, 20000, 1))
+    rng.shuffle(cands)
+    exp = [expected(cd) for cd in cands]
+    say(len(cands), "candidates;", sum(v for v, _ in exp), "probable primes expected")
+    s = gpu.MrScheduler(BASES, max_active=4, chunk=64)
+    s.submit(cands, tags=list(range(100, 100 + len(cands))))
+    res = s.run_all()
+    st = s.stats()
+    say("chunks", st.chunks, "admin batches", st.admin_batches, "pools", st.n_pools,
+        "hbm MiB", st.hbm_bytes >> 20)
+    assert len(res) == len(cands)
+    got = {r.tag - 100: (r.verdict, r.rounds) for r in res}
+    for i, cd in enumerate(cands):
+        assert got[i] == exp[i], (cd, got[i], exp[i])
+    for r in res:
+        assert r.t_submit <= r.t_admit <= r.t_done
+    assert st.completed == len(cands) and st.active == 0 and st.pending == 0
+    assert st.n_pools == 3
+    s.close()
+
+
+def test_batch_call_default_pool(gpu):
+    rng = random.Random(11)
+    cands = mixed_candidates(rng, 1000, 60) + PRIMES_1000
+    exp = [expected(cd) for cd in cands]
+    t0 = time.perf_counter()
+    v, r = gpu.mr_test_batch(cands, BASES)
+    say(len(cands), "x 1000 digits in", round(time.perf_counter() - t0, 3), "s")
+    assert list(zip(v, r)) == exp
+
+
+def test_serving_submission_while_running(gpu):
+    """Requests arrive while earlier ones run (non-blocking pumps)."""
+    rng = random.Random(13)
+    waves = [mixed_candidates(rng, 2000, 5) for _ in range(4)]
+    waves[2] += PRIMES_1000[:1]
+    s = gpu.MrScheduler([2, 3], max_active=6, chunk=32)
+    res, tag = [], 0
+    all_cands = []
+    for w in waves:
+        s.submit(w, tags=list(range(tag, tag + len(w))))
+        all_cands += w
+        tag += len(w)
+        for _ in range(3):
+            s.pump(False)
+            res += s.results()
+            s.pump(True)
+            res += s.results()
+    res += s.run_all()
+    assert sorted(r.tag for r in res) == list(range(len(all_cands)))
+    for r in res:
+        assert (r.verdict, r.rounds) == expected(all_cands[r.tag], [2, 3])
+    s.close()
+
+
+def test_invalid_submission_rejected(gpu):
+    s = gpu.MrScheduler(BASES, max_active=2)
+    with pytest.raises(ValueError):
+        s.submit([(10, 20, 1, 1, 1, 0)])           # N < 0
+    with pytest.raises(ValueError):
+        s.submit([(200000, 5, 1, 1, 1, 0)])        # beyond 2^15 transforms
+    assert s.stats().submitted == 0
+    s.close()
