@@ -70,11 +70,23 @@ def cpu_baseline(primes: np.ndarray, triple, budget_s: float) -> dict:
                        f"oracle/td_oracle.c restating trial_div_cpu.c")}
 
 
+def mr_ops_per_squaring(logm: int) -> int:
+    """INT32 VALU ops of one Montgomery squaring step on one candidate, by
+    the transform-count model T(M) = (M/2) log2(M) 12 per transform and
+    prime, ops = 2 (6 T + 24 M) + 60 M (two primes, six transforms each,
+    pointwise and CRT/carry per coefficient), at this build's transform
+    length M = 2^logm (24-bit limbs: 2^14 at 50 000 digits)."""
+    M = 1 << logm
+    T = (M // 2) * logm * 12
+    return 2 * (6 * T + 24 * M) + 60 * M
+
+
 def mr_leg(pf, n_bits: int) -> dict:
     """Batched Miller-Rabin at 50 000 digits (workload 3: 128 seeded
     candidates and their decimal reversals, built on the device): time the
-    first n_bits squaring steps of a base-2 round over the whole batch and
-    scale to the full exponent length.  Reported beside the headline."""
+    first n_bits squaring steps of a base-2 round over the whole batch with
+    HIP events on the context's stream and scale to the full exponent
+    length."""
     cfg = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())["3"]
     cands = [tuple(x) + (w,) for x in cfg["candidates"] for w in (0, 1)]
     t0 = time.perf_counter()
@@ -83,15 +95,188 @@ def mr_leg(pf, n_bits: int) -> dict:
     ctx.time_squarings(2, 64)                         # warm-up
     ms = ctx.time_squarings(2, n_bits)
     bits = 166096                                     # squarings per 50 000-digit test
-    test_s = ms / 1e3 / n_bits * bits
+    step_s = ms / 1e3 / n_bits
+    test_s = step_s * bits
+    ops = mr_ops_per_squaring(ctx.log_len) * len(cands)
+    achieved = ops / step_s / 1e12
     res = {"workload": "256 bigints (128 N + 128 rev N) at 50 000 digits, base 2",
            "limbs": ctx.limbs, "transform_len": 1 << ctx.log_len,
-           "us_per_squaring_batch": ms * 1e3 / n_bits,
+           "us_per_squaring_batch": step_s * 1e6,
            "tests_per_hour_extrapolated": 3600.0 * len(cands) / test_s,
            "sample": f"{n_bits} of {bits} squaring steps timed (HIP events), scaled",
-           "setup_s": setup_s}
+           "setup_s": setup_s,
+           "roofline": {"bound": "valu", "achieved": achieved, "peak": INT32_VALU_PEAK_TOPS,
+                        "unit": "Tops/s", "frac": achieved / INT32_VALU_PEAK_TOPS,
+                        "kernel": f"exp_kernel<{ctx.log_len}>",
+                        "ops_per_squaring_step": ops}}
     ctx.close()
     return res
+
+
+def workload3(pf, args) -> dict:
+    """Workload 3 on its own line: tests/hour of the batched path at 50 000
+    digits, extrapolated from the timed squaring steps (every candidate of
+    the workload is composite at base 2, so one round per bigint)."""
+    leg = mr_leg(pf, args.mr_bits or 2048)
+    return {
+        "metric": "Miller-Rabin tests/hour (batched, 50 000 digits)",
+        "value": leg["tests_per_hour_extrapolated"], "unit": "tests/hour", "n_gpus": 1,
+        "steps": 1, "warmup": 1, "ms_per_step": leg["us_per_squaring_batch"] / 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded candidates, tests/golden/configs.json workload 3)",
+        "config": {"workload": leg["workload"], "transform_len": leg["transform_len"],
+                   "sample": leg["sample"], "parallelism": "single GPU"},
+        "roofline": leg["roofline"],
+        "device": pf.device_info(),
+    }
+
+
+def _mr_cands(rows, coin=False):
+    """(a, b, c, k, j, which) tuples: each row as N and rev(N), or (coin)
+    one of the two, chosen by the row's sixth field."""
+    out = []
+    for r in rows:
+        if coin:
+            out.append(tuple(r[:5]) + (int(r[5]),))
+        else:
+            out += [tuple(r[:5]) + (0,), tuple(r[:5]) + (1,)]
+    return out
+
+
+def _progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def workload4(pf, args) -> dict:
+    """512 candidates, a log-uniform in [50 000, 100 000] digits, each as N
+    and rev(N) (1 024 bigints across the 2^14 and 2^15 transform buckets),
+    bases 2, 3, 5, 7 with retirement at the first failing base, through the
+    continuous-batching scheduler.  Wall time from submission to the last
+    verdict."""
+    cfg = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())["4"]
+    cands = _mr_cands(cfg["candidates"])
+    if args.limit:
+        cands = cands[: args.limit]
+    s = pf.MrScheduler(cfg["bases"], chunk=args.chunk)
+    t0 = time.perf_counter()
+    s.submit(cands)
+    res, last = [], t0
+    while True:
+        left = s.pump(True)
+        res += s.results()
+        now = time.perf_counter()
+        if now - last > 30:
+            _progress(f"workload 4: {len(res)}/{len(cands)} decided after {now - t0:.0f} s")
+            last = now
+        if left == 0:
+            break
+    wall = time.perf_counter() - t0
+    st = s.stats()
+    s.close()
+    rounds = sum(r.rounds for r in res)
+    tests_h = 3600.0 * rounds / wall
+    return {
+        "metric": "Miller-Rabin tests/hour (batched, mixed sizes)",
+        "value": tests_h, "unit": "tests/hour", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded candidates, tests/golden/configs.json workload 4)",
+        "config": {"workload": f"{len(cands)} bigints (N and rev N), a log-uniform in "
+                               f"[50000, 100000] digits, bases 2,3,5,7, retire on first failure",
+                   "candidates": len(cands), "parallelism": "single GPU"},
+        "candidates_per_hour": 3600.0 * len(cands) / wall,
+        "rounds": rounds, "probable_primes": sum(r.verdict for r in res),
+        "scheduler": {"chunks": st.chunks, "admin_batches": st.admin_batches,
+                      "slot_steps": st.squarings, "hbm_bytes": st.hbm_bytes,
+                      "max_active": st.max_active, "pools": st.n_pools, "chunk": args.chunk},
+        "device": pf.device_info(),
+    }
+
+
+def capacity_estimate(pf, rows, bases_n=1) -> dict:
+    """Requests/s the scheduler sustains on this request mix: measured
+    seconds per squaring step of a full slot pool (one candidate per compute
+    unit) at each transform length, times the mean exponent length of each
+    bucket, assuming one round per request (composites retire at base 2)."""
+    cus = pf.device_info()["cus"]
+    per_step = {}
+    for digits in (50000, 100000):
+        probe = [(digits, 20000, 300, 4001, 17, w) for w in (0, 1)] * (cus // 2)
+        ctx = pf.MrContext(candidates=probe)
+        ctx.time_squarings(2, 16)
+        per_step[ctx.log_len] = ctx.time_squarings(2, 256) / 1e3 / 256
+        ctx.close()
+    busy = 0.0
+    for r in rows:
+        bits = int(r[0] * 3.3219280948873623)
+        L = (bits + 2 + 23) // 24
+        logm = 14 if 2 * L + 2 <= 1 << 14 else 15
+        busy += bits * per_step[logm]
+    mean_s = busy / len(rows)               # one slot busy per request
+    return {"requests_per_s": cus / mean_s, "mean_test_s": mean_s,
+            "s_per_step": {str(1 << k): v for k, v in per_step.items()}}
+
+
+def workload5(pf, args) -> dict:
+    """Serving: requests of tests/golden/configs.json workload 5 (a uniform in
+    [50 000, 100 000] digits, N or rev(N) by the row's coin, bases 2, 3, 5, 7)
+    arrive as a seeded Poisson process at `load` times the measured capacity
+    for `serve_seconds`; the scheduler admits them first come, first served
+    into free slots.  Reports time-to-verdict percentiles over the requests
+    that arrived in the window (all are drained)."""
+    cfg = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())["5"]
+    rows = cfg["requests"]
+    cap = capacity_estimate(pf, rows)
+    rate = args.load * cap["requests_per_s"]
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(rows)))
+    n_req = int(np.searchsorted(arrivals, args.serve_seconds))
+    cands = _mr_cands(rows[:n_req], coin=True)
+    _progress(f"workload 5: capacity {cap['requests_per_s']:.3f} req/s, load {args.load}, "
+              f"{n_req} requests over {args.serve_seconds} s")
+    s = pf.MrScheduler(cfg["bases"], chunk=args.chunk)
+    t0 = time.perf_counter()
+    mono0 = time.clock_gettime(time.CLOCK_MONOTONIC)
+    nxt, res, last = 0, [], t0
+    while True:
+        now = time.perf_counter() - t0
+        k = nxt
+        while k < n_req and arrivals[k] <= now:
+            k += 1
+        if k > nxt:
+            s.submit(cands[nxt:k], tags=list(range(nxt, k)))
+            nxt = k
+        left = s.pump(True)
+        res += s.results()
+        if time.perf_counter() - last > 30:
+            last = time.perf_counter()
+            _progress(f"workload 5: t={last - t0:.0f} s, {nxt} arrived, {len(res)} decided")
+        if nxt == n_req and left == 0:
+            break
+        if nxt < n_req and left == 0:        # idle until the next arrival
+            time.sleep(max(0.0, min(0.05, arrivals[nxt] - (time.perf_counter() - t0))))
+    wall = time.perf_counter() - t0
+    st = s.stats()
+    s.close()
+    ttv = np.array([r.t_done - (mono0 + arrivals[r.tag]) for r in res])
+    rounds = sum(r.rounds for r in res)
+    return {
+        "metric": "Miller-Rabin serving tests/hour",
+        "value": 3600.0 * rounds / wall, "unit": "tests/hour", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u32",
+        "data": "synthetic (seeded requests, tests/golden/configs.json workload 5)",
+        "config": {"workload": f"serving: Poisson arrivals at {args.load} x capacity for "
+                               f"{args.serve_seconds} s, a in [50000, 100000] digits, bases 2,3,5,7",
+                   "requests": n_req, "load": args.load, "parallelism": "single GPU"},
+        "p50_s": float(np.percentile(ttv, 50)), "p99_s": float(np.percentile(ttv, 99)),
+        "mean_s": float(ttv.mean()), "max_s": float(ttv.max()),
+        "capacity": cap, "arrival_rate_per_s": rate,
+        "scheduler": {"chunks": st.chunks, "admin_batches": st.admin_batches,
+                      "max_active": st.max_active, "pools": st.n_pools,
+                      "hbm_bytes": st.hbm_bytes, "hbm_budget": 64 << 30},
+        "device": pf.device_info(),
+    }
 
 
 def main() -> int:
@@ -105,6 +290,14 @@ def main() -> int:
                     help="squaring steps timed per Miller-Rabin sample (0 = skip the MR leg)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one plain copy kernel of this many MiB (counter calibration)")
+    ap.add_argument("--workload", type=int, default=2, choices=(2, 3, 4, 5),
+                    help="2: trial-division headline; 3: batched Miller-Rabin at 50 000 digits; "
+                         "4: batched Miller-Rabin of mixed sizes; 5: Miller-Rabin serving")
+    ap.add_argument("--limit", type=int, default=0, help="workload 4: first N bigints only")
+    ap.add_argument("--chunk", type=int, default=256, help="squaring steps per scheduler chunk")
+    ap.add_argument("--load", type=float, default=0.70, help="workload 5: arrival rate / capacity")
+    ap.add_argument("--serve-seconds", type=float, default=300.0,
+                    help="workload 5: length of the arrival window")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not "
@@ -112,6 +305,10 @@ def main() -> int:
         return 2
 
     pf = _load("pf_amd", ROOT / "prime-finder_amd" / "pf_amd.py")
+    if args.workload in (3, 4, 5):
+        out = {3: workload3, 4: workload4, 5: workload5}[args.workload](pf, args)
+        print(json.dumps(out), flush=True)
+        return 0
     L = pf.lib()
     assert L.cuda_get_device_count() >= 1, "no HIP device"
     cfg = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())["2"]

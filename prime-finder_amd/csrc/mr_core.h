@@ -87,6 +87,29 @@ __device__ __forceinline__ uint32_t mont(uint32_t a, uint32_t b, uint32_t p, uin
     return min(r, r - p);
 }
 
+// ------------------------------------------------------------- LDS layout
+// A transform buffer of M words occupies MP = M + M/64 words of LDS: one
+// pad word after every 64.  When each thread walks its own contiguous run
+// of C = 8, 16 or 32 coefficients, the 64 lanes of a wave (stride C words)
+// would share 64/C banks; with the pad they reach 64 different banks.  64
+// lanes reading 64 consecutive coefficients still reach 64 consecutive
+// words.
+__device__ __forceinline__ int px(int n) { return n + (n >> 6); }
+template <int LOGM>
+struct Lds {
+    static constexpr int M = 1 << LOGM;
+    static constexpr int MP = M + (M >> 6);
+};
+
+// Padded address of base + j*H for a butterfly half-width H: one shift-add
+// per group when H is a multiple of 64, else per element.
+template <int H>
+__device__ __forceinline__ int pxo(int pbase, int base, int j)
+{
+    if constexpr (H % 64 == 0) return pbase + j * (H + H / 64);
+    else return px(base + j * H);
+}
+
 // ------------------------------------------------------------- tables
 // Per (prime, M): tw[i] = w^i and itw[i] = w^-i for i < M/2 with their Shoup
 // quotients, w a primitive M-th root of unity.  Layout of one table block:
@@ -101,112 +124,6 @@ struct Twiddles {
 };
 
 // ------------------------------------------------------------- transforms
-// Forward DIF, natural -> bit-reversed, on NP buffers X + q*M (prime q0+q)
-// in the same passes (one barrier per radix-4 pass for all of them).
-template <int LOGM, int NP>
-__device__ void ntt_fwd(uint32_t *X, const Twiddles &T, int q0)
-{
-    constexpr int M = 1 << LOGM;
-    constexpr int GROUPS = M / 4;
-    const int tid = tid_();
-    int s = 0;
-    for (; s + 1 < LOGM; s += 2) {
-        const int h2 = M >> (s + 2);
-#pragma unroll
-        for (int q = 0; q < NP; q++) {
-            uint32_t *Y = X + q * M;
-            const uint32_t p = prime(q0 + q);
-            const uint32_t *tw = T.tw(q0 + q), *twq = T.twq(q0 + q);
-            for (int g = tid; g < GROUPS; g += THREADS) {
-                const int pos = g & (h2 - 1);
-                const int base = ((g - pos) << 2) + pos;
-                uint32_t x0 = Y[base], x1 = Y[base + h2], x2 = Y[base + 2 * h2], x3 = Y[base + 3 * h2];
-                const int ia = pos << s, ib = (pos + h2) << s, ic = pos << (s + 1);
-                // stage s: (x0, x2), (x1, x3)
-                uint32_t y0 = add_mod(x0, x2, p);
-                uint32_t y2 = shoup(x0 - x2 + p, tw[ia], twq[ia], p);
-                uint32_t y1 = add_mod(x1, x3, p);
-                uint32_t y3 = shoup(x1 - x3 + p, tw[ib], twq[ib], p);
-                // stage s+1: (y0, y1), (y2, y3)
-                const uint32_t wc = tw[ic], wcq = twq[ic];
-                Y[base] = add_mod(y0, y1, p);
-                Y[base + h2] = shoup(y0 - y1 + p, wc, wcq, p);
-                Y[base + 2 * h2] = add_mod(y2, y3, p);
-                Y[base + 3 * h2] = shoup(y2 - y3 + p, wc, wcq, p);
-            }
-        }
-        __syncthreads();
-    }
-    if (s < LOGM) {   // odd LOGM: final radix-2 stage, half = 1, twiddle 1
-#pragma unroll
-        for (int q = 0; q < NP; q++) {
-            uint32_t *Y = X + q * M;
-            const uint32_t p = prime(q0 + q);
-            for (int g = tid; g < M / 2; g += THREADS) {
-                uint32_t x0 = Y[2 * g], x1 = Y[2 * g + 1];
-                Y[2 * g] = add_mod(x0, x1, p);
-                Y[2 * g + 1] = sub_mod(x0, x1, p);
-            }
-        }
-        __syncthreads();
-    }
-}
-
-// Inverse DIT, bit-reversed -> natural, result scaled by M.
-template <int LOGM, int NP>
-__device__ void ntt_inv(uint32_t *X, const Twiddles &T, int q0)
-{
-    constexpr int M = 1 << LOGM;
-    constexpr int GROUPS = M / 4;
-    const int tid = tid_();
-    int t = 0;
-    if (LOGM & 1) {   // first radix-2 stage, half = 1, twiddle 1
-#pragma unroll
-        for (int q = 0; q < NP; q++) {
-            uint32_t *Y = X + q * M;
-            const uint32_t p = prime(q0 + q);
-            for (int g = tid; g < M / 2; g += THREADS) {
-                uint32_t x0 = Y[2 * g], x1 = Y[2 * g + 1];
-                Y[2 * g] = add_mod(x0, x1, p);
-                Y[2 * g + 1] = sub_mod(x0, x1, p);
-            }
-        }
-        __syncthreads();
-        t = 1;
-    }
-    for (; t + 1 < LOGM; t += 2) {
-        const int h = 1 << t;
-#pragma unroll
-        for (int q = 0; q < NP; q++) {
-            uint32_t *Y = X + q * M;
-            const uint32_t p = prime(q0 + q);
-            const uint32_t *itw = T.itw(q0 + q), *itwq = T.itwq(q0 + q);
-            for (int g = tid; g < GROUPS; g += THREADS) {
-                const int pos = g & (h - 1);
-                const int base = ((g - pos) << 2) + pos;
-                uint32_t x0 = Y[base], x1 = Y[base + h], x2 = Y[base + 2 * h], x3 = Y[base + 3 * h];
-                // stage t (half h): (x0, x1), (x2, x3), twiddle w^-(pos * M/(2h))
-                const int ia = pos << (LOGM - 1 - t);
-                const uint32_t wa = itw[ia], waq = itwq[ia];
-                uint32_t b1 = shoup(x1, wa, waq, p);
-                uint32_t b3 = shoup(x3, wa, waq, p);
-                uint32_t y0 = add_mod(x0, b1, p), y1 = sub_mod(x0, b1, p);
-                uint32_t y2 = add_mod(x2, b3, p), y3 = sub_mod(x2, b3, p);
-                // stage t+1 (half 2h): (y0, y2) with w^-(pos*M/(4h)),
-                // (y1, y3) with w^-((pos+h)*M/(4h))
-                const int ib = pos << (LOGM - 2 - t), ic = (pos + h) << (LOGM - 2 - t);
-                uint32_t c2 = shoup(y2, itw[ib], itwq[ib], p);
-                uint32_t c3 = shoup(y3, itw[ic], itwq[ic], p);
-                Y[base] = add_mod(y0, c2, p);
-                Y[base + 2 * h] = sub_mod(y0, c2, p);
-                Y[base + h] = add_mod(y1, c3, p);
-                Y[base + 3 * h] = sub_mod(y1, c3, p);
-            }
-        }
-        __syncthreads();
-    }
-}
-
 // ------------------------------------------------- fused transform passes
 // The same transforms as ntt_fwd / ntt_inv, organised for throughput:
 //  * per-thread trip counts are compile-time constants;
@@ -217,24 +134,92 @@ __device__ void ntt_inv(uint32_t *X, const Twiddles &T, int q0)
 //    consecutive elements (two for odd LOGM) with the constant twiddles 1 and
 //    w^(+-M/4), so they are fused with the pointwise step in registers.
 
+// Workgroup barrier for LDS hand-offs only: waits for this wave's LDS
+// operations, not for its outstanding global loads (__syncthreads waits for
+// both), so loads issued for the next pass stay in flight.  The memory
+// clobber keeps the compiler from moving memory accesses across it.  Only
+// for sections that write no global memory.
+__device__ __forceinline__ void lds_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Per-thread twiddles of one radix-4 pass (wa, wa', wb, wb', wc, wc' with
+// their Shoup quotients) for NP primes.
+template <int NP>
+struct TwSet {
+    uint32_t v[NP][6];
+};
+
+// A pass whose butterfly half-width is <= THREADS gives each thread one
+// position inside its block, hence one twiddle set for all its groups.
+template <int LOGM, int S>
+__device__ constexpr bool dif_hoisted() { return ((1 << LOGM) >> (S + 2)) <= THREADS; }
+template <int LOGM, int T0>
+__device__ constexpr bool dit_hoisted() { return (1 << T0) <= THREADS; }
+
+template <int LOGM, int S, int NP>
+__device__ __forceinline__ TwSet<NP> dif_tw(const Twiddles &T, int q0)
+{
+    constexpr int H2 = (1 << LOGM) >> (S + 2);
+    const int pos = tid_() & (H2 - 1);
+    const int ia = pos << S, ib = (pos + H2) << S, ic = pos << (S + 1);
+    TwSet<NP> t;
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+        const uint32_t *tw = T.tw(q0 + q), *twq = T.twq(q0 + q);
+        t.v[q][0] = tw[ia]; t.v[q][1] = twq[ia];
+        t.v[q][2] = tw[ib]; t.v[q][3] = twq[ib];
+        t.v[q][4] = tw[ic]; t.v[q][5] = twq[ic];
+    }
+    return t;
+}
+
+template <int LOGM, int T0, int NP>
+__device__ __forceinline__ TwSet<NP> dit_tw(const Twiddles &T, int q0)
+{
+    constexpr int H = 1 << T0;
+    const int pos = tid_() & (H - 1);
+    const int ia = pos << (LOGM - 1 - T0), ib = pos << (LOGM - 2 - T0), ic = (pos + H) << (LOGM - 2 - T0);
+    TwSet<NP> t;
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+        const uint32_t *itw = T.itw(q0 + q), *itwq = T.itwq(q0 + q);
+        t.v[q][0] = itw[ia]; t.v[q][1] = itwq[ia];
+        t.v[q][2] = itw[ib]; t.v[q][3] = itwq[ib];
+        t.v[q][4] = itw[ic]; t.v[q][5] = itwq[ic];
+    }
+    return t;
+}
+
+// Element k of prime q's set, q a loop counter of a non-unrolled loop:
+// selects instead of indexing (indexing would put the set in scratch).
+template <int NP>
+__device__ __forceinline__ uint32_t tw_pick(const TwSet<NP> &t, int q, int k)
+{
+    uint32_t v = t.v[0][k];
+#pragma unroll
+    for (int r = 1; r < NP; r++) v = q == r ? t.v[r][k] : v;
+    return v;
+}
+
 // Regular forward radix-4 pass starting at stage S.
 template <int LOGM, int S, int NP>
-__device__ __forceinline__ void dif_pass(uint32_t *X, const Twiddles &T, int q0)
+__device__ __forceinline__ void dif_pass(uint32_t *X, const Twiddles &T, int q0, const TwSet<NP> &cur)
 {
-    constexpr int M = 1 << LOGM, H2 = M >> (S + 2), GROUPS = M / 4;
+    constexpr int M = 1 << LOGM, H2 = M >> (S + 2), GROUPS = M / 4, MP = Lds<LOGM>::MP;
     constexpr int IT = GROUPS >= THREADS ? GROUPS / THREADS : 1;
     const int tid = tid_();
     if (GROUPS >= THREADS || tid < GROUPS) {
 #pragma unroll 1
         for (int q = 0; q < NP; q++) {
-            uint32_t *Y = X + q * M;
+            uint32_t *Y = X + q * MP;
             const uint32_t p = prime(q0 + q);
             const uint32_t *tw = T.tw(q0 + q), *twq = T.twq(q0 + q);
             uint32_t wa = 0, waq = 0, wb = 0, wbq = 0, wc = 0, wcq = 0;
             if constexpr (H2 <= THREADS) {
-                const int pos = tid & (H2 - 1);
-                const int ia = pos << S, ib = (pos + H2) << S, ic = pos << (S + 1);
-                wa = tw[ia]; waq = twq[ia]; wb = tw[ib]; wbq = twq[ib]; wc = tw[ic]; wcq = twq[ic];
+                wa = tw_pick(cur, q, 0); waq = tw_pick(cur, q, 1); wb = tw_pick(cur, q, 2);
+                wbq = tw_pick(cur, q, 3); wc = tw_pick(cur, q, 4); wcq = tw_pick(cur, q, 5);
             }
 #pragma unroll
             for (int it = 0; it < IT; it++) {
@@ -244,41 +229,41 @@ __device__ __forceinline__ void dif_pass(uint32_t *X, const Twiddles &T, int q0)
                     const int ia = pos << S, ib = (pos + H2) << S, ic = pos << (S + 1);
                     wa = tw[ia]; waq = twq[ia]; wb = tw[ib]; wbq = twq[ib]; wc = tw[ic]; wcq = twq[ic];
                 }
-                const int base = ((g - pos) << 2) + pos;
-                const uint32_t x0 = Y[base], x1 = Y[base + H2], x2 = Y[base + 2 * H2], x3 = Y[base + 3 * H2];
+                const int base = ((g - pos) << 2) + pos, pb = px(base);
+                const int a0 = pxo<H2>(pb, base, 0), a1 = pxo<H2>(pb, base, 1),
+                          a2 = pxo<H2>(pb, base, 2), a3 = pxo<H2>(pb, base, 3);
+                const uint32_t x0 = Y[a0], x1 = Y[a1], x2 = Y[a2], x3 = Y[a3];
                 const uint32_t y0 = add_mod(x0, x2, p);
                 const uint32_t y2 = shoup(x0 - x2 + p, wa, waq, p);
                 const uint32_t y1 = add_mod(x1, x3, p);
                 const uint32_t y3 = shoup(x1 - x3 + p, wb, wbq, p);
-                Y[base] = add_mod(y0, y1, p);
-                Y[base + H2] = shoup(y0 - y1 + p, wc, wcq, p);
-                Y[base + 2 * H2] = add_mod(y2, y3, p);
-                Y[base + 3 * H2] = shoup(y2 - y3 + p, wc, wcq, p);
+                Y[a0] = add_mod(y0, y1, p);
+                Y[a1] = shoup(y0 - y1 + p, wc, wcq, p);
+                Y[a2] = add_mod(y2, y3, p);
+                Y[a3] = shoup(y2 - y3 + p, wc, wcq, p);
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // Regular inverse radix-4 pass starting at stage T0 (half 2^T0).
 template <int LOGM, int T0, int NP>
-__device__ __forceinline__ void dit_pass(uint32_t *X, const Twiddles &T, int q0)
+__device__ __forceinline__ void dit_pass(uint32_t *X, const Twiddles &T, int q0, const TwSet<NP> &cur)
 {
-    constexpr int M = 1 << LOGM, H = 1 << T0, GROUPS = M / 4;
+    constexpr int M = 1 << LOGM, H = 1 << T0, GROUPS = M / 4, MP = Lds<LOGM>::MP;
     constexpr int IT = GROUPS >= THREADS ? GROUPS / THREADS : 1;
     const int tid = tid_();
     if (GROUPS >= THREADS || tid < GROUPS) {
 #pragma unroll 1
         for (int q = 0; q < NP; q++) {
-            uint32_t *Y = X + q * M;
+            uint32_t *Y = X + q * MP;
             const uint32_t p = prime(q0 + q);
             const uint32_t *itw = T.itw(q0 + q), *itwq = T.itwq(q0 + q);
             uint32_t wa = 0, waq = 0, wb = 0, wbq = 0, wc = 0, wcq = 0;
             if constexpr (H <= THREADS) {
-                const int pos = tid & (H - 1);
-                const int ia = pos << (LOGM - 1 - T0), ib = pos << (LOGM - 2 - T0),
-                          ic = (pos + H) << (LOGM - 2 - T0);
-                wa = itw[ia]; waq = itwq[ia]; wb = itw[ib]; wbq = itwq[ib]; wc = itw[ic]; wcq = itwq[ic];
+                wa = tw_pick(cur, q, 0); waq = tw_pick(cur, q, 1); wb = tw_pick(cur, q, 2);
+                wbq = tw_pick(cur, q, 3); wc = tw_pick(cur, q, 4); wcq = tw_pick(cur, q, 5);
             }
 #pragma unroll
             for (int it = 0; it < IT; it++) {
@@ -289,42 +274,52 @@ __device__ __forceinline__ void dit_pass(uint32_t *X, const Twiddles &T, int q0)
                               ic = (pos + H) << (LOGM - 2 - T0);
                     wa = itw[ia]; waq = itwq[ia]; wb = itw[ib]; wbq = itwq[ib]; wc = itw[ic]; wcq = itwq[ic];
                 }
-                const int base = ((g - pos) << 2) + pos;
-                const uint32_t x0 = Y[base], x1 = Y[base + H], x2 = Y[base + 2 * H], x3 = Y[base + 3 * H];
+                const int base = ((g - pos) << 2) + pos, pb = px(base);
+                const int a0 = pxo<H>(pb, base, 0), a1 = pxo<H>(pb, base, 1),
+                          a2 = pxo<H>(pb, base, 2), a3 = pxo<H>(pb, base, 3);
+                const uint32_t x0 = Y[a0], x1 = Y[a1], x2 = Y[a2], x3 = Y[a3];
                 const uint32_t b1 = shoup(x1, wa, waq, p);
                 const uint32_t b3 = shoup(x3, wa, waq, p);
                 const uint32_t y0 = add_mod(x0, b1, p), y1 = sub_mod(x0, b1, p);
                 const uint32_t y2 = add_mod(x2, b3, p), y3 = sub_mod(x2, b3, p);
                 const uint32_t c2 = shoup(y2, wb, wbq, p);
                 const uint32_t c3 = shoup(y3, wc, wcq, p);
-                Y[base] = add_mod(y0, c2, p);
-                Y[base + 2 * H] = sub_mod(y0, c2, p);
-                Y[base + H] = add_mod(y1, c3, p);
-                Y[base + 3 * H] = sub_mod(y1, c3, p);
+                Y[a0] = add_mod(y0, c2, p);
+                Y[a2] = sub_mod(y0, c2, p);
+                Y[a1] = add_mod(y1, c3, p);
+                Y[a3] = sub_mod(y1, c3, p);
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
-// Forward passes S, S+2, ... while the pass is not the last one.
+// Forward passes S, S+2, ... while the pass is not the last one.  The
+// next pass's twiddles are loaded before this pass computes, so their
+// latency hides behind it (the pass barriers wait for LDS only).
 template <int LOGM, int S, int NP>
-__device__ __forceinline__ void dif_regular(uint32_t *X, const Twiddles &T, int q0)
+__device__ __forceinline__ void dif_regular(uint32_t *X, const Twiddles &T, int q0, const TwSet<NP> &cur)
 {
     constexpr int LAST = (LOGM & 1) ? LOGM - 1 : LOGM - 2;   // first stage of the final pass
     if constexpr (S < LAST) {
-        dif_pass<LOGM, S, NP>(X, T, q0);
-        dif_regular<LOGM, S + 2, NP>(X, T, q0);
+        constexpr int S2 = S + 2;
+        TwSet<NP> nxt{};
+        if constexpr (S2 < LAST && dif_hoisted<LOGM, S2>()) nxt = dif_tw<LOGM, S2, NP>(T, q0);
+        dif_pass<LOGM, S, NP>(X, T, q0, cur);
+        dif_regular<LOGM, S2, NP>(X, T, q0, nxt);
     }
 }
 
 // Inverse passes after the first one.
 template <int LOGM, int T0, int NP>
-__device__ __forceinline__ void dit_regular(uint32_t *X, const Twiddles &T, int q0)
+__device__ __forceinline__ void dit_regular(uint32_t *X, const Twiddles &T, int q0, const TwSet<NP> &cur)
 {
     if constexpr (T0 + 1 < LOGM) {
-        dit_pass<LOGM, T0, NP>(X, T, q0);
-        dit_regular<LOGM, T0 + 2, NP>(X, T, q0);
+        constexpr int T2 = T0 + 2;
+        TwSet<NP> nxt{};
+        if constexpr (T2 + 1 < LOGM && dit_hoisted<LOGM, T2>()) nxt = dit_tw<LOGM, T2, NP>(T, q0);
+        dit_pass<LOGM, T0, NP>(X, T, q0, cur);
+        dit_regular<LOGM, T2, NP>(X, T, q0, nxt);
     }
 }
 
@@ -336,11 +331,11 @@ template <int LOGM, int NP, int MODE>
 __device__ __forceinline__ void middle(uint32_t *X, const Twiddles &T, int q0, const uint32_t *B,
                                        const uint32_t *np_tab)
 {
-    constexpr int M = 1 << LOGM;
+    constexpr int M = 1 << LOGM, MP = Lds<LOGM>::MP;
     const int tid = tid_();
 #pragma unroll
     for (int q = 0; q < NP; q++) {
-        uint32_t *Y = X + q * M;
+        uint32_t *Y = X + q * MP;
         const uint32_t p = prime(q0 + q), np = np_tab[q0 + q];
         const uint32_t *Bq = B ? B + (size_t)(q0 + q) * M : nullptr;
         if constexpr ((LOGM & 1) == 0) {
@@ -353,7 +348,8 @@ __device__ __forceinline__ void middle(uint32_t *X, const Twiddles &T, int q0, c
 #pragma unroll
                 for (int it = 0; it < IT; it++) {
                     const int g = tid + it * THREADS;
-                    uint4 v = *reinterpret_cast<uint4 *>(Y + 4 * g);
+                    uint32_t *Yg = Y + px(4 * g);      // 4 words inside one 64-word block
+                    uint4 v = make_uint4(Yg[0], Yg[1], Yg[2], Yg[3]);
                     // DIF stages LOGM-2 (twiddles 1, w^(M/4)) and LOGM-1 (twiddle 1)
                     const uint32_t y0 = add_mod(v.x, v.z, p), y2 = sub_mod(v.x, v.z, p);
                     const uint32_t y1 = add_mod(v.y, v.w, p);
@@ -376,7 +372,7 @@ __device__ __forceinline__ void middle(uint32_t *X, const Twiddles &T, int q0, c
                     } else {
                         v.x = z0; v.y = z1; v.z = z2; v.w = z3;
                     }
-                    *reinterpret_cast<uint4 *>(Y + 4 * g) = v;
+                    Yg[0] = v.x; Yg[1] = v.y; Yg[2] = v.z; Yg[3] = v.w;
                 }
             }
         } else {
@@ -386,7 +382,8 @@ __device__ __forceinline__ void middle(uint32_t *X, const Twiddles &T, int q0, c
 #pragma unroll
                 for (int it = 0; it < IT; it++) {
                     const int g = tid + it * THREADS;
-                    uint2 v = *reinterpret_cast<uint2 *>(Y + 2 * g);
+                    uint32_t *Yg = Y + px(2 * g);
+                    uint2 v = make_uint2(Yg[0], Yg[1]);
                     uint32_t z0 = add_mod(v.x, v.y, p), z1 = sub_mod(v.x, v.y, p);
                     if constexpr (MODE != 0) {
                         uint2 bb = make_uint2(z0, z1);
@@ -397,12 +394,12 @@ __device__ __forceinline__ void middle(uint32_t *X, const Twiddles &T, int q0, c
                     } else {
                         v.x = z0; v.y = z1;
                     }
-                    *reinterpret_cast<uint2 *>(Y + 2 * g) = v;
+                    Yg[0] = v.x; Yg[1] = v.y;
                 }
             }
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // Cyclic convolution of the NP buffers with themselves (B null) or with the
@@ -411,16 +408,22 @@ template <int LOGM, int NP>
 __device__ __forceinline__ void convolve(uint32_t *X, const Twiddles &T, int q0, const uint32_t *B,
                                          const uint32_t *np_tab)
 {
-    dif_regular<LOGM, 0, NP>(X, T, q0);
+    constexpr int T0 = (LOGM & 1) ? 1 : 2;     // first inverse pass after middle
+    TwSet<NP> f{}, d{};
+    if constexpr (dif_hoisted<LOGM, 0>()) f = dif_tw<LOGM, 0, NP>(T, q0);
+    dif_regular<LOGM, 0, NP>(X, T, q0, f);
+    if constexpr (T0 + 1 < LOGM && dit_hoisted<LOGM, T0>()) d = dit_tw<LOGM, T0, NP>(T, q0);
     middle<LOGM, NP, 1>(X, T, q0, B, np_tab);
-    dit_regular<LOGM, (LOGM & 1) ? 1 : 2, NP>(X, T, q0);
+    dit_regular<LOGM, T0, NP>(X, T, q0, d);
 }
 
 // Forward transform only (bit-reversed output).
 template <int LOGM, int NP>
 __device__ __forceinline__ void forward(uint32_t *X, const Twiddles &T, int q0, const uint32_t *np_tab)
 {
-    dif_regular<LOGM, 0, NP>(X, T, q0);
+    TwSet<NP> f{};
+    if constexpr (dif_hoisted<LOGM, 0>()) f = dif_tw<LOGM, 0, NP>(T, q0);
+    dif_regular<LOGM, 0, NP>(X, T, q0, f);
     middle<LOGM, NP, 0>(X, T, q0, nullptr, np_tab);
 }
 

@@ -24,6 +24,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include <time.h>
+#include <algorithm>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -77,7 +78,7 @@ struct G {
 
 template <int LOGM>
 struct Smem {
-    uint32_t X[G<LOGM>::NP * G<LOGM>::M];
+    uint32_t X[G<LOGM>::NP * Lds<LOGM>::MP];      // padded transform buffers
     Scratch sc;
     uint32_t bcast[8];
 };
@@ -105,6 +106,29 @@ __device__ __forceinline__ void store_vec(const uint32_t (&v)[C], uint32_t *dst,
     }
 }
 
+// A thread's own elements (positions t*C + i in [lo, hi)) parked in global
+// memory lane-interleaved, element i of thread t at buf[i * THREADS + t], so
+// that each store or load instruction of a wave touches consecutive words;
+// only the same thread reads them back.  load_own zero-fills outside
+// [lo, hi).
+template <int C>
+__device__ __forceinline__ void save_own(const uint32_t (&v)[C], uint32_t *buf, int lo, int hi)
+{
+    const int t = tid_(), p0 = t * C;
+#pragma unroll
+    for (int i = 0; i < C; i++)
+        if (p0 + i >= lo && p0 + i < hi) buf[i * THREADS + t] = v[i];
+}
+
+template <int C>
+__device__ __forceinline__ void load_own(uint32_t (&v)[C], const uint32_t *buf, int lo, int hi)
+{
+    const int t = tid_(), p0 = t * C;
+#pragma unroll
+    for (int i = 0; i < C; i++)
+        v[i] = (p0 + i >= lo && p0 + i < hi) ? buf[i * THREADS + t] : 0u;
+}
+
 template <int C>
 __device__ __forceinline__ void zero_below(uint32_t (&v)[C], int lim)
 {
@@ -119,47 +143,27 @@ __device__ __forceinline__ void zero_below(uint32_t (&v)[C], int lim)
 template <int LOGM, int NPW>
 __device__ __forceinline__ void stage_in(uint32_t *X, const uint32_t (&v)[G<LOGM>::C], int off, int len)
 {
-    constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
+    constexpr int C = G<LOGM>::C, MP = Lds<LOGM>::MP;
     const int p0 = tid_() * C;
     if (off != 0) {
 #pragma unroll
         for (int q = 0; q < NPW; q++)
 #pragma unroll
-            for (int i = 0; i < C; i++) X[q * M + p0 + i] = 0u;
+            for (int i = 0; i < C; i++) X[q * MP + px(p0 + i)] = 0u;
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < C; i++) {
             const int idx = p0 + i - off;
             if (idx >= 0 && idx < len)
 #pragma unroll
-                for (int q = 0; q < NPW; q++) X[q * M + idx] = v[i];
+                for (int q = 0; q < NPW; q++) X[q * MP + px(idx)] = v[i];
         }
     } else {
 #pragma unroll
         for (int i = 0; i < C; i++) {
             const uint32_t x = (p0 + i < len) ? v[i] : 0u;
 #pragma unroll
-            for (int q = 0; q < NPW; q++) X[q * M + p0 + i] = x;
-        }
-    }
-    __syncthreads();
-}
-
-// Pointwise step on NPW buffers for primes q0..: square (B == nullptr) or
-// Montgomery product with B[(q0+q)*M + i] (an operand kept times 2^32).
-template <int LOGM, int NPW>
-__device__ __forceinline__ void pointwise(uint32_t *X, const uint32_t *B, int q0, const Consts &K)
-{
-    constexpr int M = G<LOGM>::M;
-#pragma unroll
-    for (int q = 0; q < NPW; q++) {
-        const uint32_t p = prime(q0 + q), np = K.np[q0 + q];
-        uint32_t *Y = X + q * M;
-        if (B) {
-            const uint32_t *Bq = B + (size_t)(q0 + q) * M;
-            for (int i = tid_(); i < M; i += THREADS) Y[i] = mont(Y[i], Bq[i], p, np);
-        } else {
-            for (int i = tid_(); i < M; i += THREADS) { const uint32_t y = Y[i]; Y[i] = mont(y, y, p, np); }
+            for (int q = 0; q < NPW; q++) X[q * MP + px(p0 + i)] = x;
         }
     }
     __syncthreads();
@@ -181,19 +185,19 @@ __device__ void tproduct(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
     if constexpr (NP == 2) {
         stage_in<LOGM, 2>(sm.X, a, off, len);
         convolve<LOGM, 2>(sm.X, T, 0, Bf, K.np);
-        const uint32_t *X0 = sm.X, *X1 = sm.X + M;
-        crt_carry<C>(out, [&](int i) { return X0[p0 + i]; }, [&](int i) { return X1[p0 + i]; },
+        const uint32_t *X0 = sm.X, *X1 = sm.X + Lds<LOGM>::MP;
+        crt_carry<C>(out, [&](int i) { return X0[px(p0 + i)]; }, [&](int i) { return X1[px(p0 + i)]; },
                      k, keep, sm.sc);
     } else {
         stage_in<LOGM, 1>(sm.X, a, off, len);
         convolve<LOGM, 1>(sm.X, T, 0, Bf, K.np);
 #pragma unroll
-        for (int i = 0; i < C; i++) ysave[p0 + i] = sm.X[p0 + i];
+        for (int i = 0; i < C; i++) ysave[i * THREADS + tid_()] = sm.X[px(p0 + i)];   // coalesced
         __syncthreads();
         stage_in<LOGM, 1>(sm.X, a, off, len);
         convolve<LOGM, 1>(sm.X, T, 1, Bf, K.np);
         const uint32_t *X1 = sm.X;
-        crt_carry<C>(out, [&](int i) { return ysave[p0 + i]; }, [&](int i) { return X1[p0 + i]; },
+        crt_carry<C>(out, [&](int i) { return ysave[i * THREADS + tid_()]; }, [&](int i) { return X1[px(p0 + i)]; },
                      k, keep, sm.sc);
     }
 }
@@ -211,7 +215,7 @@ __device__ void to_fft(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
         for (int q = 0; q < NP; q++) {
             const uint32_t p = prime(q0 + q), np = K.np[q0 + q], r2 = K.r2[q0 + q];
             for (int i = tid_(); i < M; i += THREADS)
-                dst[(size_t)(q0 + q) * M + i] = mont(sm.X[q * M + i], r2, p, np);
+                dst[(size_t)(q0 + q) * M + i] = mont(sm.X[q * Lds<LOGM>::MP + px(i)], r2, p, np);
         }
         __syncthreads();
     }
@@ -232,7 +236,7 @@ __device__ __forceinline__ void reduce_once(uint32_t (&r)[C], const uint32_t *Nl
 }
 
 // r <- r * s mod N for r < N, 1 <= s < 2^24 (double-and-add of modular
-// additions).  The multiplicand waits in LDS (stash, C * THREADS words) so
+// additions).  The multiplicand waits in LDS (stash, padded, MP words) so
 // that at most three limb vectors are live.
 template <int C>
 __device__ void mulsmall_modN(uint32_t (&r)[C], uint32_t s, const uint32_t *Nl, int L, int off,
@@ -240,7 +244,7 @@ __device__ void mulsmall_modN(uint32_t (&r)[C], uint32_t s, const uint32_t *Nl, 
 {
     const int p0 = tid_() * C;
 #pragma unroll
-    for (int i = 0; i < C; i++) stash[p0 + i] = r[i];       // own slots only: no barrier
+    for (int i = 0; i < C; i++) stash[px(p0 + i)] = r[i];   // own slots only: no barrier
     const int top = 31 - __clz((int)s);
     for (int b = top - 1; b >= 0; b--) {
         add_vec<C>(r, r, 0u, sc);                            // 2r < 2N
@@ -248,7 +252,7 @@ __device__ void mulsmall_modN(uint32_t (&r)[C], uint32_t s, const uint32_t *Nl, 
         if ((s >> b) & 1u) {
             uint32_t t[C];
 #pragma unroll
-            for (int i = 0; i < C; i++) t[i] = stash[p0 + i];
+            for (int i = 0; i < C; i++) t[i] = stash[px(p0 + i)];
             add_vec<C>(r, t, 0u, sc);
             reduce_once<C>(r, Nl, L, off, sc);
         }
@@ -284,13 +288,13 @@ __device__ void montmul(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, cons
 #pragma unroll
             for (int i = 0; i < C; i++) nz |= (p0 + i < L) && r[i] != 0u;
             cbit = block_any(nz, sm.sc);
-            store_vec<C>(r, park, L + 1, L);                   // T_hi
+            save_own<C>(r, park, L, 2 * L + 1);                // T_hi
         }
     }
     zero_below<C>(r, L);
     {
         uint32_t t[C];
-        load_vec<C>(t, park, L + 1, L);
+        load_own<C>(t, park, L, 2 * L + 1);
         // + [T_lo != 0] at position L: T_hi's lowest limb is below 2^24, so
         // adding 1 cannot overflow the 32-bit lane before add_vec normalises
 #pragma unroll
@@ -334,21 +338,21 @@ __device__ void mul_small_vec(uint32_t (&x)[C], uint32_t s, Scratch &sc)
     }
 }
 
-// x <- x * 2^e through the LDS buffer X (M words).
+// x <- x * 2^e through the padded LDS buffer X (MP words).
 template <int C>
 __device__ void shl_vec(uint32_t (&x)[C], long e, uint32_t *X, int M)
 {
     const int p0 = tid_() * C;
 #pragma unroll
-    for (int i = 0; i < C; i++) X[p0 + i] = x[i];
+    for (int i = 0; i < C; i++) X[px(p0 + i)] = x[i];
     __syncthreads();
     const long ls = e / 24;
     const int bs = (int)(e % 24);
 #pragma unroll
     for (int i = 0; i < C; i++) {
         const long k = (long)(p0 + i) - ls;
-        const uint32_t hi = (k >= 0 && k < M) ? X[k] : 0u;
-        const uint32_t lo = (k - 1 >= 0 && k - 1 < M) ? X[k - 1] : 0u;
+        const uint32_t hi = (k >= 0 && k < M) ? X[px((int)k)] : 0u;
+        const uint32_t lo = (k - 1 >= 0 && k - 1 < M) ? X[px((int)k - 1)] : 0u;
         x[i] = bs ? (((hi << bs) | (lo >> (24 - bs))) & LIMB_MASK) : hi;
     }
     __syncthreads();
@@ -575,7 +579,7 @@ __global__ __launch_bounds__(THREADS) void round_start_kernel(Dev D, uint32_t ba
 {
     constexpr int C = G<LOGM>::C;
     __shared__ Scratch sc;
-    __shared__ uint32_t stash[G<LOGM>::M];
+    __shared__ uint32_t stash[Lds<LOGM>::MP];
     __shared__ uint32_t go;
     const int c = slot_id(D);
     if (mode != 0) {
@@ -1414,7 +1418,8 @@ int pool_new(mr_sched *S, int logm, Pool **out)
     p->x = ctx_new(1, n, S->device);
     int rc = p->x ? 0 : -2;
     // numbers of up to (2^logm - 2) / 2 limbs: the pool's transform length
-    if (rc == 0) rc = ctx_alloc_all(p->x, 24l * (((1l << logm) - 2) / 2), logm);
+    const long lmax = std::min<long>(((1l << logm) - 2) / 2, MAX_LIMBS);
+    if (rc == 0) rc = ctx_alloc_all(p->x, 24l * lmax, logm);
     if (rc == 0 && p->x->logm != logm) rc = -1;
     auto dev = [&](void **q, size_t b) { if (rc == 0 && ctx_alloc(p->x, q, b) != 0) rc = -2; };
     dev((void **)&p->d_list_run, 4 * (size_t)n);

@@ -56,7 +56,13 @@ enum {
 };
 
 // ctrl words
-enum { CT_SMALL_COUNT = 0, CT_ERROR = 1, CT_WORDS = 4 };
+enum { CT_SMALL_COUNT = 0, CT_ERROR = 1, CT_ITEMS = 2, CT_ITEM_OVF = 3, CT_WORDS = 4 };
+
+// Row-kernel work items: (small prime, kind, part), each at most TD_CH
+// marking steps, so that lanes of one wave get comparable work.
+constexpr uint32_t TD_CH       = 32;
+constexpr uint32_t TD_ITEM_CAP = 32768;   // > 2 * 6542 + 20000/32 * sum(1/p)
+enum { TI_N = 0, TI_REV = 1, TI_WHOLE = 2 };
 
 __device__ __forceinline__ uint32_t td_pow10(int e)
 {
@@ -186,6 +192,35 @@ __device__ __forceinline__ void td_lds_mark(uint32_t *lds, uint32_t bit)
     atomicOr(&lds[bit >> 5], 1u << (bit & 31u));
 }
 
+// Every mark of one small prime (entry e) in row k: the normal solve, then
+// each rev digit group separately (any a - c).
+template <int DJ>
+__device__ __forceinline__ void td_row_prime(const uint32_t *__restrict__ e, uint32_t k, int lk,
+                                             uint32_t rk, uint32_t rowbit, uint32_t J,
+                                             uint32_t *lds)
+{
+    const uint32_t p = e[TT_P * TD_SMALL_CAP], magic = e[TT_MAGIC * TD_SMALL_CAP];
+    const uint32_t flags = e[TT_FLAGS * TD_SMALL_CAP];
+    if (flags & 1u) {
+        const uint32_t u = pf_submod(e[TT_UN0 * TD_SMALL_CAP],
+                                     td_mod(k * e[TT_B * TD_SMALL_CAP], p, magic), p);
+        for (uint32_t j = u + 1u; j <= J; j += p) td_lds_mark(lds, rowbit + j);
+    }
+#pragma unroll
+    for (int lj = 1; lj <= DJ; lj++) {
+        if (!((flags >> (1 + (lk - 1) * 4 + (lj - 1))) & 1u)) continue;
+        const uint32_t ev = e[(TT_E + (lk - 1) * 4 + (lj - 1)) * TD_SMALL_CAP];
+        const uint32_t u = pf_submod(e[(TT_CM + lj - 1) * TD_SMALL_CAP],
+                                     td_mod(rk * ev, p, magic), p);
+        const uint32_t lim = td_pow10(lj);
+        for (uint32_t r = u + 1u; r < lim; r += p) {
+            if (r % 10u == 0u) continue;
+            const uint32_t j = td_reverse(r, lj);
+            if (j <= J) td_lds_mark(lds, rowbit + j);
+        }
+    }
+}
+
 template <int DJ>
 __global__ __launch_bounds__(TD_BLOCK)
 void td_rows_kernel(const uint32_t *__restrict__ tbl, const uint32_t *__restrict__ ctrl,
@@ -209,26 +244,130 @@ void td_rows_kernel(const uint32_t *__restrict__ tbl, const uint32_t *__restrict
         const uint32_t rk = td_reverse(k, lk);
         // LDS-relative bit of (k, j) is rowbit + j.
         const uint32_t rowbit = (k - 1u) * J - (w0 << 5) - 1u;
-        for (uint32_t i = slot; i < count; i += nslots) {
-            const uint32_t *e = tbl + i;
-            const uint32_t p = e[TT_P * TD_SMALL_CAP], magic = e[TT_MAGIC * TD_SMALL_CAP];
-            const uint32_t flags = e[TT_FLAGS * TD_SMALL_CAP];
-            if (flags & 1u) {
-                const uint32_t u = pf_submod(e[TT_UN0 * TD_SMALL_CAP],
-                                             td_mod(k * e[TT_B * TD_SMALL_CAP], p, magic), p);
-                for (uint32_t j = u + 1u; j <= J; j += p) td_lds_mark(lds, rowbit + j);
-            }
-#pragma unroll
-            for (int lj = 1; lj <= DJ; lj++) {
-                if (!((flags >> (1 + (lk - 1) * 4 + (lj - 1))) & 1u)) continue;
-                const uint32_t ev = e[(TT_E + (lk - 1) * 4 + (lj - 1)) * TD_SMALL_CAP];
-                const uint32_t u = pf_submod(e[(TT_CM + lj - 1) * TD_SMALL_CAP],
-                                             td_mod(rk * ev, p, magic), p);
-                const uint32_t lim = td_pow10(lj);
-                for (uint32_t r = u + 1u; r < lim; r += p) {
-                    if (r % 10u == 0u) continue;
-                    const uint32_t j = td_reverse(r, lj);
-                    if (j <= J) td_lds_mark(lds, rowbit + j);
+        for (uint32_t i = slot; i < count; i += nslots)
+            td_row_prime<DJ>(tbl + i, k, lk, rk, rowbit, J, lds);
+    }
+    __syncthreads();
+    for (uint32_t w = threadIdx.x; w < nw; w += TD_BLOCK) {
+        const uint32_t v = lds[w];
+        if (v) atomicOr(&bits[w0 + w], v);
+    }
+}
+
+// Work items of the balanced row kernel (one workgroup): per small prime p,
+// ceil(ceil(J/p) / TD_CH) normal-solve parts when the solve is active and
+// ceil(ceil((10^DJ - 1)/p) / TD_CH) merged-rev parts; p = 2 and 5 (whose
+// powers of ten vanish, so digit groups are gated one by one) keep one
+// whole-prime item.  Item = index | kind << 16 | part << 18.
+template <int DJ>
+__global__ __launch_bounds__(1024)
+void td_items_kernel(const uint32_t *__restrict__ tbl, uint32_t *__restrict__ ctrl, uint32_t J,
+                     uint32_t *__restrict__ items)
+{
+    __shared__ uint32_t scan[1024];
+    const uint32_t count = min(ctrl[CT_SMALL_COUNT], TD_SMALL_CAP);
+    const uint32_t per = (count + 1023u) / 1024u;
+    const uint32_t i0 = min(threadIdx.x * per, count), i1 = min(i0 + per, count);
+    const uint32_t lim = td_pow10(DJ);
+    auto parts = [&](uint32_t i, uint32_t &nn, uint32_t &nr) {
+        const uint32_t p = tbl[TT_P * TD_SMALL_CAP + i];
+        if (p == 2u || p == 5u) { nn = 0; nr = 0; return true; }
+        const uint32_t flags = tbl[TT_FLAGS * TD_SMALL_CAP + i];
+        nn = (flags & 1u) ? ((J + p - 1u) / p + TD_CH - 1u) / TD_CH : 0u;
+        nr = ((lim - 1u + p - 1u) / p + TD_CH - 1u) / TD_CH;
+        return false;
+    };
+    uint32_t mine = 0;
+    for (uint32_t i = i0; i < i1; i++) {
+        uint32_t nn, nr;
+        mine += parts(i, nn, nr) ? 1u : nn + nr;
+    }
+    scan[threadIdx.x] = mine;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024u; d <<= 1) {          // inclusive scan
+        const uint32_t v = threadIdx.x >= d ? scan[threadIdx.x - d] : 0u;
+        __syncthreads();
+        scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const uint32_t total = scan[1023];
+    if (threadIdx.x == 0) {
+        ctrl[CT_ITEMS] = total;
+        ctrl[CT_ITEM_OVF] = total > TD_ITEM_CAP ? 1u : 0u;
+    }
+    if (total > TD_ITEM_CAP) return;
+    uint32_t at = scan[threadIdx.x] - mine;
+    for (uint32_t i = i0; i < i1; i++) {
+        uint32_t nn, nr;
+        if (parts(i, nn, nr)) {
+            items[at++] = i | (uint32_t)TI_WHOLE << 16;
+            continue;
+        }
+        for (uint32_t q = 0; q < nn; q++) items[at++] = i | (uint32_t)TI_N << 16 | q << 18;
+        for (uint32_t q = 0; q < nr; q++) items[at++] = i | (uint32_t)TI_REV << 16 | q << 18;
+    }
+}
+
+// Row kernel with merged rev residues (a - c >= DJ): lanes spread over
+// (row, work item) instead of (row, prime), so the many marks of the
+// smallest primes are shared by many lanes.  rdec is the DJ-digit decode
+// table.  Falls back to whole primes if the item list overflowed.
+template <int DJ>
+__global__ __launch_bounds__(TD_BLOCK)
+void td_rows_bal_kernel(const uint32_t *__restrict__ tbl, const uint32_t *__restrict__ ctrl,
+                        const uint32_t *__restrict__ items, const uint16_t *__restrict__ rdec,
+                        uint32_t K, uint32_t J, uint32_t *__restrict__ bits)
+{
+    extern __shared__ uint32_t lds[];
+    const uint32_t k0 = blockIdx.x * TD_ROWS + 1u;
+    const uint32_t k_last = min(k0 + TD_ROWS - 1u, K);
+    const uint32_t w0 = ((k0 - 1u) * J) >> 5;
+    const uint32_t nw = ((k_last * J - 1u) >> 5) - w0 + 1u;
+    for (uint32_t w = threadIdx.x; w < nw; w += TD_BLOCK) lds[w] = 0u;
+    __syncthreads();
+
+    const uint32_t row = threadIdx.x % TD_ROWS;
+    const uint32_t slot = threadIdx.x / TD_ROWS;
+    constexpr uint32_t nslots = TD_BLOCK / TD_ROWS;
+    const uint32_t k = k0 + row;
+    if (k <= K) {
+        const int lk = td_ndigits(k);
+        const uint32_t rk = td_reverse(k, lk);
+        const uint32_t rowbit = (k - 1u) * J - (w0 << 5) - 1u;
+        if (ctrl[CT_ITEM_OVF]) {
+            const uint32_t count = min(ctrl[CT_SMALL_COUNT], TD_SMALL_CAP);
+            for (uint32_t i = slot; i < count; i += nslots)
+                td_row_prime<DJ>(tbl + i, k, lk, rk, rowbit, J, lds);
+        } else {
+            const uint32_t n_items = ctrl[CT_ITEMS];
+            const uint32_t lim = td_pow10(DJ);
+            const int fbit = 1 + (lk - 1) * 4 + (DJ - 1);
+            for (uint32_t it = slot; it < n_items; it += nslots) {
+                const uint32_t item = items[it];
+                const uint32_t *e = tbl + (item & 0xFFFFu);
+                const uint32_t kind = (item >> 16) & 3u, part = item >> 18;
+                if (kind == TI_WHOLE) {
+                    td_row_prime<DJ>(e, k, lk, rk, rowbit, J, lds);
+                    continue;
+                }
+                const uint32_t p = e[TT_P * TD_SMALL_CAP], magic = e[TT_MAGIC * TD_SMALL_CAP];
+                if (kind == TI_N) {
+                    const uint32_t u = pf_submod(e[TT_UN0 * TD_SMALL_CAP],
+                                                 td_mod(k * e[TT_B * TD_SMALL_CAP], p, magic), p);
+                    uint32_t j = u + 1u + part * TD_CH * p;
+                    for (uint32_t t = 0; t < TD_CH && j <= J; t++, j += p)
+                        td_lds_mark(lds, rowbit + j);
+                } else {
+                    const uint32_t flags = e[TT_FLAGS * TD_SMALL_CAP];
+                    if (!((flags >> fbit) & 1u)) continue;
+                    const uint32_t ev = e[(TT_E + (lk - 1) * 4 + (DJ - 1)) * TD_SMALL_CAP];
+                    const uint32_t u = pf_submod(e[(TT_CM + DJ - 1) * TD_SMALL_CAP],
+                                                 td_mod(rk * ev, p, magic), p);
+                    uint32_t r = u + 1u + part * TD_CH * p;
+                    for (uint32_t t = 0; t < TD_CH && r < lim; t++, r += p) {
+                        const uint32_t j = rdec[r];
+                        if (j <= J) td_lds_mark(lds, rowbit + j);
+                    }
                 }
             }
         }
@@ -438,6 +577,7 @@ struct TdScratch {
     uint32_t *tbl = nullptr;     // [TD_SMALL_CAP * TD_TBL]
     uint32_t *ctrl = nullptr;    // [CT_WORDS]
     uint16_t *rdec = nullptr;    // [TD_RDEC_SIZE]
+    uint32_t *items = nullptr;   // [TD_ITEM_CAP] row-kernel work items
 };
 
 TdScratch g_scratch[TD_MAX_DEV];
@@ -453,6 +593,7 @@ int td_scratch(TdScratch **out)
         if (hipMalloc(&s.ctrl, CT_WORDS * sizeof(uint32_t)) != hipSuccess) return -1;
         if (hipMemset(s.ctrl, 0, CT_WORDS * sizeof(uint32_t)) != hipSuccess) return -1;
         if (hipMalloc(&s.rdec, TD_RDEC_SIZE * sizeof(uint16_t)) != hipSuccess) return -1;
+        if (hipMalloc(&s.items, TD_ITEM_CAP * sizeof(uint32_t)) != hipSuccess) return -1;
         td_rdec_kernel<<<(TD_RDEC_SIZE + 255u) / 256u, 256>>>(s.rdec);
         if (hipDeviceSynchronize() != hipSuccess) return -1;
     }
@@ -467,7 +608,13 @@ void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long
 {
     const uint32_t row_groups = (K + TD_ROWS - 1u) / TD_ROWS;
     const size_t lds = ((size_t)TD_ROWS * J / 32u + 2u) * sizeof(uint32_t);
-    td_rows_kernel<DJ><<<row_groups, TD_BLOCK, lds, st>>>(s.tbl, s.ctrl, K, J, d_bits);
+    if (a - c >= DJ) {
+        td_items_kernel<DJ><<<1, 1024, 0, st>>>(s.tbl, s.ctrl, J, s.items);
+        td_rows_bal_kernel<DJ><<<row_groups, TD_BLOCK, lds, st>>>(
+            s.tbl, s.ctrl, s.items, s.rdec + TD_RDEC_OFF[DJ], K, J, d_bits);
+    } else {
+        td_rows_kernel<DJ><<<row_groups, TD_BLOCK, lds, st>>>(s.tbl, s.ctrl, K, J, d_bits);
+    }
     if (ev0) (void)hipEventRecord(ev0, st);
     // One merged rev residue per k needs every j digit group's exponent
     // a-c-lj >= 0; otherwise track the groups separately.
