@@ -203,6 +203,34 @@ __device__ __forceinline__ uint32_t tw_pick(const TwSet<NP> &t, int q, int k)
     return v;
 }
 
+// Group handled by a lane in a pass of butterfly half-width H < 64.  With
+// the natural order (group = thread index + k * THREADS) a wave's 64 lanes
+// cover 64/H consecutive blocks of 4H words, whose padded addresses fall on
+// 16 to 32 of the 64 banks (2- to 4-way conflicts).  Rotating lane
+// bits 4..5 (bit 5 when H = 32) up above the block bits that fall on
+// distinct banks makes the wave's blocks lie 16 blocks apart, where the pad
+// word per 64 shifts each block to its own banks: every access of these
+// passes is then conflict-free.  A bit permutation of the group index, so
+// still every group exactly once; the position inside a block (low bits)
+// and hence the twiddles are unchanged.
+template <int H, int GROUPS>
+__device__ __forceinline__ int pass_group(int g)
+{
+    constexpr int h = __builtin_ctz(H);
+    if constexpr (H >= 64) {
+        return g;
+    } else {
+        constexpr int lo = h <= 4 ? 4 : 5, w = h <= 4 ? h + 2 : 5, k = h <= 4 ? 2 : 1;
+        if constexpr ((1 << (lo + w)) > GROUPS) {
+            return g;
+        } else {
+            const int F = (g >> lo) & ((1 << w) - 1);
+            const int F2 = (F >> k) | ((F & ((1 << k) - 1)) << (w - k));
+            return (g & ~(((1 << w) - 1) << lo)) | (F2 << lo);
+        }
+    }
+}
+
 // Regular forward radix-4 pass starting at stage S.
 template <int LOGM, int S, int NP>
 __device__ __forceinline__ void dif_pass(uint32_t *X, const Twiddles &T, int q0, const TwSet<NP> &cur)
@@ -223,7 +251,7 @@ __device__ __forceinline__ void dif_pass(uint32_t *X, const Twiddles &T, int q0,
             }
 #pragma unroll
             for (int it = 0; it < IT; it++) {
-                const int g = tid + it * THREADS;
+                const int g = pass_group<H2, GROUPS>(tid + it * THREADS);
                 const int pos = g & (H2 - 1);
                 if constexpr (H2 > THREADS) {
                     const int ia = pos << S, ib = (pos + H2) << S, ic = pos << (S + 1);
@@ -267,7 +295,7 @@ __device__ __forceinline__ void dit_pass(uint32_t *X, const Twiddles &T, int q0,
             }
 #pragma unroll
             for (int it = 0; it < IT; it++) {
-                const int g = tid + it * THREADS;
+                const int g = pass_group<H, GROUPS>(tid + it * THREADS);
                 const int pos = g & (H - 1);
                 if constexpr (H > THREADS) {
                     const int ia = pos << (LOGM - 1 - T0), ib = pos << (LOGM - 2 - T0),
