@@ -36,6 +36,14 @@ DJ = 4
 OPS_PER_PRIME_K = 4 * 2
 INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12      # 39.32 Tops/s (256 CUs, wave64, 2.4 GHz)
 SMALL_LIMIT = 1 << 16                               # primes below go to the LDS row kernel
+# HBM bytes per launch of the lane kernel on this workload, measured with
+# rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 256 MiB copy kernel's
+# readings: x2.0 and x1.0): 23.3 MB read (the prime list) + 535.5 MB written
+# (memory-side atomic ORs of the ~1.7e7 hits).
+TD_TRAFFIC_BYTES = 23.3e6 + 535.5e6
+# Same for one batched squaring step of the 50 000-digit Miller-Rabin batch
+# (256 bigints, 2^14 transforms): 82.4 MB read + 7.3 MB written.
+MR_TRAFFIC_BYTES_PER_STEP = 82.4e6 + 7.3e6
 
 
 def _load(name, path):
@@ -107,6 +115,10 @@ def mr_leg(pf, n_bits: int) -> dict:
            "setup_s": setup_s,
            "roofline": {"bound": "valu", "achieved": achieved, "peak": INT32_VALU_PEAK_TOPS,
                         "unit": "Tops/s", "frac": achieved / INT32_VALU_PEAK_TOPS,
+                        "traffic": MR_TRAFFIC_BYTES_PER_STEP if ctx.log_len == 14 else None,
+                        "traffic_note": "HBM bytes per batched squaring step (rocprofv3 "
+                                        "FETCH_SIZE + WRITE_SIZE); algorithmic 67.1e6 "
+                                        "(NTT(N), NTT(N') per bigint)",
                         "kernel": f"exp_kernel<{ctx.log_len}>",
                         "ops_per_squaring_step": ops}}
     ctx.close()
@@ -385,7 +397,10 @@ def main() -> int:
                    "parallelism": "single GPU"},
         "roofline": {"bound": "valu", "achieved": achieved, "peak": INT32_VALU_PEAK_TOPS,
                      "unit": "Tops/s", "frac": achieved / INT32_VALU_PEAK_TOPS,
-                     "traffic": None,
+                     "traffic": TD_TRAFFIC_BYTES,
+                     "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE); "
+                                     "algorithmic 25.1e6 (prime list + bitmap), the rest are "
+                                     "hit atomics; 0.11 TB/s, far below the 8 TB/s roof",
                      "kernel": "td_large_kernel<4,merged>",
                      "kernel_avg_ms": avg_lane_s * 1e3,
                      "ops_per_launch": n_large * K * OPS_PER_PRIME_K},

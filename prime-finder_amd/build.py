@@ -2,7 +2,9 @@
 
 Every ``csrc/*.hip`` file is compiled by hipcc for gfx950 only, every
 ``csrc/*.c`` file by gcc, and the objects are linked into one shared library
-whose exported C symbols are declared in ``include/*.h``.
+whose exported C symbols are declared in ``include/*.h``.  ``mr_cpp_check``
+(``csrc/mr_cpp_check.cpp``, a C++ caller of ``include/batch_mod_ctx.hpp``)
+is linked against it.
 """
 from __future__ import annotations
 
@@ -62,6 +64,10 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *objs])
     shutil.move(str(tmp), str(LIB))
+    # C++ caller of include/batch_mod_ctx.hpp (the reference's runner surface)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}",
+          str(CSRC / "mr_cpp_check.cpp"), "-o", str(LIB.parent / "mr_cpp_check"),
+          f"-L{LIB.parent}", "-lpfamd", "-Wl,-rpath,$ORIGIN"])
     return LIB
 
 

@@ -101,13 +101,16 @@ struct Lds {
     static constexpr int MP = M + (M >> 6);
 };
 
-// Padded address of base + j*H for a butterfly half-width H: one shift-add
-// per group when H is a multiple of 64, else per element.
+// Padded address of element base + j*H of a radix-4 group with butterfly
+// half-width H, from one per-group register: the padded base when H is a
+// multiple of 64 (the j offsets are then constants), else the plain base.
 template <int H>
-__device__ __forceinline__ int pxo(int pbase, int base, int j)
+__device__ __forceinline__ int grp_base(int base) { return H % 64 == 0 ? px(base) : base; }
+template <int H>
+__device__ __forceinline__ int grp_addr(int b, int j)
 {
-    if constexpr (H % 64 == 0) return pbase + j * (H + H / 64);
-    else return px(base + j * H);
+    if constexpr (H % 64 == 0) return b + j * (H + H / 64);
+    else return px(b + j * H);
 }
 
 // ------------------------------------------------------------- tables
@@ -192,17 +195,6 @@ __device__ __forceinline__ TwSet<NP> dit_tw(const Twiddles &T, int q0)
     return t;
 }
 
-// Element k of prime q's set, q a loop counter of a non-unrolled loop:
-// selects instead of indexing (indexing would put the set in scratch).
-template <int NP>
-__device__ __forceinline__ uint32_t tw_pick(const TwSet<NP> &t, int q, int k)
-{
-    uint32_t v = t.v[0][k];
-#pragma unroll
-    for (int r = 1; r < NP; r++) v = q == r ? t.v[r][k] : v;
-    return v;
-}
-
 // Group handled by a lane in a pass of butterfly half-width H < 64.  With
 // the natural order (group = thread index + k * THREADS) a wave's 64 lanes
 // cover 64/H consecutive blocks of 4H words, whose padded addresses fall on
@@ -231,91 +223,132 @@ __device__ __forceinline__ int pass_group(int g)
     }
 }
 
-// Regular forward radix-4 pass starting at stage S.
+// Regular forward radix-4 pass starting at stage S.  A thread's groups go
+// in batches of GB per prime (at most 8 groups, 32 words, in registers):
+// every word of a batch, for all NP primes, is read before the first
+// butterfly result is written.  The compiler cannot tell that one group's
+// stores miss the next group's loads, so loads placed after stores would
+// wait for them and each wave would alternate between LDS latency and
+// arithmetic.
 template <int LOGM, int S, int NP>
 __device__ __forceinline__ void dif_pass(uint32_t *X, const Twiddles &T, int q0, const TwSet<NP> &cur)
 {
     constexpr int M = 1 << LOGM, H2 = M >> (S + 2), GROUPS = M / 4, MP = Lds<LOGM>::MP;
     constexpr int IT = GROUPS >= THREADS ? GROUPS / THREADS : 1;
+    constexpr int MAXG = LOGM >= 15 ? 2 : 8;     // 2^15: registers hold 32 limbs already
+    constexpr int GB = NP * IT > MAXG ? MAXG / NP : IT;
     const int tid = tid_();
     if (GROUPS >= THREADS || tid < GROUPS) {
-#pragma unroll 1
-        for (int q = 0; q < NP; q++) {
-            uint32_t *Y = X + q * MP;
-            const uint32_t p = prime(q0 + q);
-            const uint32_t *tw = T.tw(q0 + q), *twq = T.twq(q0 + q);
-            uint32_t wa = 0, waq = 0, wb = 0, wbq = 0, wc = 0, wcq = 0;
-            if constexpr (H2 <= THREADS) {
-                wa = tw_pick(cur, q, 0); waq = tw_pick(cur, q, 1); wb = tw_pick(cur, q, 2);
-                wbq = tw_pick(cur, q, 3); wc = tw_pick(cur, q, 4); wcq = tw_pick(cur, q, 5);
+#pragma unroll
+        for (int i0 = 0; i0 < IT; i0 += GB) {
+            int ad[GB][4];
+            uint32_t x[NP][GB][4];
+#pragma unroll
+            for (int ib = 0; ib < GB; ib++) {
+                const int g = pass_group<H2, GROUPS>(tid + (i0 + ib) * THREADS);
+                const int pos = g & (H2 - 1);
+                const int gb = grp_base<H2>(((g - pos) << 2) + pos);
+#pragma unroll
+                for (int j = 0; j < 4; j++) ad[ib][j] = grp_addr<H2>(gb, j);
             }
 #pragma unroll
-            for (int it = 0; it < IT; it++) {
-                const int g = pass_group<H2, GROUPS>(tid + it * THREADS);
-                const int pos = g & (H2 - 1);
-                if constexpr (H2 > THREADS) {
-                    const int ia = pos << S, ib = (pos + H2) << S, ic = pos << (S + 1);
-                    wa = tw[ia]; waq = twq[ia]; wb = tw[ib]; wbq = twq[ib]; wc = tw[ic]; wcq = twq[ic];
+            for (int q = 0; q < NP; q++)
+#pragma unroll
+                for (int ib = 0; ib < GB; ib++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) x[q][ib][j] = X[q * MP + ad[ib][j]];
+#pragma unroll
+            for (int q = 0; q < NP; q++) {
+                uint32_t *Y = X + q * MP;
+                const uint32_t p = prime(q0 + q);
+                const uint32_t *tw = T.tw(q0 + q), *twq = T.twq(q0 + q);
+                uint32_t wa = 0, waq = 0, wb = 0, wbq = 0, wc = 0, wcq = 0;
+                if constexpr (H2 <= THREADS) {
+                    wa = cur.v[q][0]; waq = cur.v[q][1]; wb = cur.v[q][2];
+                    wbq = cur.v[q][3]; wc = cur.v[q][4]; wcq = cur.v[q][5];
                 }
-                const int base = ((g - pos) << 2) + pos, pb = px(base);
-                const int a0 = pxo<H2>(pb, base, 0), a1 = pxo<H2>(pb, base, 1),
-                          a2 = pxo<H2>(pb, base, 2), a3 = pxo<H2>(pb, base, 3);
-                const uint32_t x0 = Y[a0], x1 = Y[a1], x2 = Y[a2], x3 = Y[a3];
-                const uint32_t y0 = add_mod(x0, x2, p);
-                const uint32_t y2 = shoup(x0 - x2 + p, wa, waq, p);
-                const uint32_t y1 = add_mod(x1, x3, p);
-                const uint32_t y3 = shoup(x1 - x3 + p, wb, wbq, p);
-                Y[a0] = add_mod(y0, y1, p);
-                Y[a1] = shoup(y0 - y1 + p, wc, wcq, p);
-                Y[a2] = add_mod(y2, y3, p);
-                Y[a3] = shoup(y2 - y3 + p, wc, wcq, p);
+#pragma unroll
+                for (int ib = 0; ib < GB; ib++) {
+                    if constexpr (H2 > THREADS) {
+                        const int pos = pass_group<H2, GROUPS>(tid + (i0 + ib) * THREADS) & (H2 - 1);
+                        const int ia = pos << S, ibw = (pos + H2) << S, ic = pos << (S + 1);
+                        wa = tw[ia]; waq = twq[ia]; wb = tw[ibw]; wbq = twq[ibw]; wc = tw[ic]; wcq = twq[ic];
+                    }
+                    const uint32_t x0 = x[q][ib][0], x1 = x[q][ib][1], x2 = x[q][ib][2], x3 = x[q][ib][3];
+                    const uint32_t y0 = add_mod(x0, x2, p);
+                    const uint32_t y2 = shoup(x0 - x2 + p, wa, waq, p);
+                    const uint32_t y1 = add_mod(x1, x3, p);
+                    const uint32_t y3 = shoup(x1 - x3 + p, wb, wbq, p);
+                    Y[ad[ib][0]] = add_mod(y0, y1, p);
+                    Y[ad[ib][1]] = shoup(y0 - y1 + p, wc, wcq, p);
+                    Y[ad[ib][2]] = add_mod(y2, y3, p);
+                    Y[ad[ib][3]] = shoup(y2 - y3 + p, wc, wcq, p);
+                }
             }
         }
     }
     lds_barrier();
 }
 
-// Regular inverse radix-4 pass starting at stage T0 (half 2^T0).
+// Regular inverse radix-4 pass starting at stage T0 (half 2^T0); batched
+// like dif_pass.
 template <int LOGM, int T0, int NP>
 __device__ __forceinline__ void dit_pass(uint32_t *X, const Twiddles &T, int q0, const TwSet<NP> &cur)
 {
     constexpr int M = 1 << LOGM, H = 1 << T0, GROUPS = M / 4, MP = Lds<LOGM>::MP;
     constexpr int IT = GROUPS >= THREADS ? GROUPS / THREADS : 1;
+    constexpr int MAXG = LOGM >= 15 ? 2 : 8;     // 2^15: registers hold 32 limbs already
+    constexpr int GB = NP * IT > MAXG ? MAXG / NP : IT;
     const int tid = tid_();
     if (GROUPS >= THREADS || tid < GROUPS) {
-#pragma unroll 1
-        for (int q = 0; q < NP; q++) {
-            uint32_t *Y = X + q * MP;
-            const uint32_t p = prime(q0 + q);
-            const uint32_t *itw = T.itw(q0 + q), *itwq = T.itwq(q0 + q);
-            uint32_t wa = 0, waq = 0, wb = 0, wbq = 0, wc = 0, wcq = 0;
-            if constexpr (H <= THREADS) {
-                wa = tw_pick(cur, q, 0); waq = tw_pick(cur, q, 1); wb = tw_pick(cur, q, 2);
-                wbq = tw_pick(cur, q, 3); wc = tw_pick(cur, q, 4); wcq = tw_pick(cur, q, 5);
+#pragma unroll
+        for (int i0 = 0; i0 < IT; i0 += GB) {
+            int ad[GB][4];
+            uint32_t x[NP][GB][4];
+#pragma unroll
+            for (int ib = 0; ib < GB; ib++) {
+                const int g = pass_group<H, GROUPS>(tid + (i0 + ib) * THREADS);
+                const int pos = g & (H - 1);
+                const int gb = grp_base<H>(((g - pos) << 2) + pos);
+#pragma unroll
+                for (int j = 0; j < 4; j++) ad[ib][j] = grp_addr<H>(gb, j);
             }
 #pragma unroll
-            for (int it = 0; it < IT; it++) {
-                const int g = pass_group<H, GROUPS>(tid + it * THREADS);
-                const int pos = g & (H - 1);
-                if constexpr (H > THREADS) {
-                    const int ia = pos << (LOGM - 1 - T0), ib = pos << (LOGM - 2 - T0),
-                              ic = (pos + H) << (LOGM - 2 - T0);
-                    wa = itw[ia]; waq = itwq[ia]; wb = itw[ib]; wbq = itwq[ib]; wc = itw[ic]; wcq = itwq[ic];
+            for (int q = 0; q < NP; q++)
+#pragma unroll
+                for (int ib = 0; ib < GB; ib++)
+#pragma unroll
+                    for (int j = 0; j < 4; j++) x[q][ib][j] = X[q * MP + ad[ib][j]];
+#pragma unroll
+            for (int q = 0; q < NP; q++) {
+                uint32_t *Y = X + q * MP;
+                const uint32_t p = prime(q0 + q);
+                const uint32_t *itw = T.itw(q0 + q), *itwq = T.itwq(q0 + q);
+                uint32_t wa = 0, waq = 0, wb = 0, wbq = 0, wc = 0, wcq = 0;
+                if constexpr (H <= THREADS) {
+                    wa = cur.v[q][0]; waq = cur.v[q][1]; wb = cur.v[q][2];
+                    wbq = cur.v[q][3]; wc = cur.v[q][4]; wcq = cur.v[q][5];
                 }
-                const int base = ((g - pos) << 2) + pos, pb = px(base);
-                const int a0 = pxo<H>(pb, base, 0), a1 = pxo<H>(pb, base, 1),
-                          a2 = pxo<H>(pb, base, 2), a3 = pxo<H>(pb, base, 3);
-                const uint32_t x0 = Y[a0], x1 = Y[a1], x2 = Y[a2], x3 = Y[a3];
-                const uint32_t b1 = shoup(x1, wa, waq, p);
-                const uint32_t b3 = shoup(x3, wa, waq, p);
-                const uint32_t y0 = add_mod(x0, b1, p), y1 = sub_mod(x0, b1, p);
-                const uint32_t y2 = add_mod(x2, b3, p), y3 = sub_mod(x2, b3, p);
-                const uint32_t c2 = shoup(y2, wb, wbq, p);
-                const uint32_t c3 = shoup(y3, wc, wcq, p);
-                Y[a0] = add_mod(y0, c2, p);
-                Y[a2] = sub_mod(y0, c2, p);
-                Y[a1] = add_mod(y1, c3, p);
-                Y[a3] = sub_mod(y1, c3, p);
+#pragma unroll
+                for (int ib = 0; ib < GB; ib++) {
+                    if constexpr (H > THREADS) {
+                        const int pos = pass_group<H, GROUPS>(tid + (i0 + ib) * THREADS) & (H - 1);
+                        const int ia = pos << (LOGM - 1 - T0), ibw = pos << (LOGM - 2 - T0),
+                                  ic = (pos + H) << (LOGM - 2 - T0);
+                        wa = itw[ia]; waq = itwq[ia]; wb = itw[ibw]; wbq = itwq[ibw]; wc = itw[ic]; wcq = itwq[ic];
+                    }
+                    const uint32_t x0 = x[q][ib][0], x1 = x[q][ib][1], x2 = x[q][ib][2], x3 = x[q][ib][3];
+                    const uint32_t b1 = shoup(x1, wa, waq, p);
+                    const uint32_t b3 = shoup(x3, wa, waq, p);
+                    const uint32_t y0 = add_mod(x0, b1, p), y1 = sub_mod(x0, b1, p);
+                    const uint32_t y2 = add_mod(x2, b3, p), y3 = sub_mod(x2, b3, p);
+                    const uint32_t c2 = shoup(y2, wb, wbq, p);
+                    const uint32_t c3 = shoup(y3, wc, wcq, p);
+                    Y[ad[ib][0]] = add_mod(y0, c2, p);
+                    Y[ad[ib][2]] = sub_mod(y0, c2, p);
+                    Y[ad[ib][1]] = add_mod(y1, c3, p);
+                    Y[ad[ib][3]] = sub_mod(y1, c3, p);
+                }
             }
         }
     }

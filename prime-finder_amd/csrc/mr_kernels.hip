@@ -49,7 +49,7 @@ struct Dev {                    // per-batch device state (kernel argument)
     uint32_t *nm1;              // [n][stride] (N - 1) R mod N
     uint32_t *res;              // [n][stride] current residue (Montgomery form)
     uint32_t *scr;              // [n][2M] transform scratch
-    uint32_t *ysv;              // [n][M] first prime's product while the second runs (M = 2^15)
+    uint32_t *ysv;              // [n][M] at M = 2^15: first prime's product while the second runs
     int32_t *nbits;             // [n]
     int32_t *sval;              // [n] s with N - 1 = 2^s d
     int32_t *bitpos;            // [n] next exponent bit (bit index of N), < s when done
@@ -179,7 +179,7 @@ __device__ void tproduct(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
                          const uint32_t *Bf, uint32_t (&out)[G<LOGM>::C], int keep,
                          uint32_t *ysave)
 {
-    constexpr int M = G<LOGM>::M, C = G<LOGM>::C, NP = G<LOGM>::NP;
+    constexpr int C = G<LOGM>::C, NP = G<LOGM>::NP;
     const CrtConst k = Bf ? K.mul : K.sq;
     const int p0 = tid_() * C;
     if constexpr (NP == 2) {
@@ -189,6 +189,7 @@ __device__ void tproduct(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
         crt_carry<C>(out, [&](int i) { return X0[px(p0 + i)]; }, [&](int i) { return X1[px(p0 + i)]; },
                      k, keep, sm.sc);
     } else {
+        // One prime at a time; the first prime's product waits in ysave.
         stage_in<LOGM, 1>(sm.X, a, off, len);
         convolve<LOGM, 1>(sm.X, T, 0, Bf, K.np);
 #pragma unroll

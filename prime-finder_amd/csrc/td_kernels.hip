@@ -578,6 +578,8 @@ struct TdScratch {
     uint32_t *ctrl = nullptr;    // [CT_WORDS]
     uint16_t *rdec = nullptr;    // [TD_RDEC_SIZE]
     uint32_t *items = nullptr;   // [TD_ITEM_CAP] row-kernel work items
+    hipStream_t aux = nullptr;   // row kernel, concurrent with the lane kernel
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 TdScratch g_scratch[TD_MAX_DEV];
@@ -595,6 +597,9 @@ int td_scratch(TdScratch **out)
         if (hipMalloc(&s.rdec, TD_RDEC_SIZE * sizeof(uint16_t)) != hipSuccess) return -1;
         if (hipMalloc(&s.items, TD_ITEM_CAP * sizeof(uint32_t)) != hipSuccess) return -1;
         td_rdec_kernel<<<(TD_RDEC_SIZE + 255u) / 256u, 256>>>(s.rdec);
+        if (hipStreamCreateWithFlags(&s.aux, hipStreamNonBlocking) != hipSuccess) return -1;
+        if (hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) != hipSuccess) return -1;
+        if (hipEventCreateWithFlags(&s.join, hipEventDisableTiming) != hipSuccess) return -1;
         if (hipDeviceSynchronize() != hipSuccess) return -1;
     }
     *out = &s;
@@ -608,13 +613,19 @@ void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long
 {
     const uint32_t row_groups = (K + TD_ROWS - 1u) / TD_ROWS;
     const size_t lds = ((size_t)TD_ROWS * J / 32u + 2u) * sizeof(uint32_t);
-    if (a - c >= DJ) {
-        td_items_kernel<DJ><<<1, 1024, 0, st>>>(s.tbl, s.ctrl, J, s.items);
-        td_rows_bal_kernel<DJ><<<row_groups, TD_BLOCK, lds, st>>>(
+    // The row kernel (latency-bound) runs on the auxiliary stream beside the
+    // lane kernel (VALU-bound); both only OR into the bitmap.  The join
+    // before returning keeps the shared scratch (table, items) from being
+    // overwritten by the next call while the row kernel still reads it.
+    if (a - c >= DJ) td_items_kernel<DJ><<<1, 1024, 0, st>>>(s.tbl, s.ctrl, J, s.items);
+    (void)hipEventRecord(s.fork, st);
+    (void)hipStreamWaitEvent(s.aux, s.fork, 0);
+    if (a - c >= DJ)
+        td_rows_bal_kernel<DJ><<<row_groups, TD_BLOCK, lds, s.aux>>>(
             s.tbl, s.ctrl, s.items, s.rdec + TD_RDEC_OFF[DJ], K, J, d_bits);
-    } else {
-        td_rows_kernel<DJ><<<row_groups, TD_BLOCK, lds, st>>>(s.tbl, s.ctrl, K, J, d_bits);
-    }
+    else
+        td_rows_kernel<DJ><<<row_groups, TD_BLOCK, lds, s.aux>>>(s.tbl, s.ctrl, K, J, d_bits);
+    (void)hipEventRecord(s.join, s.aux);
     if (ev0) (void)hipEventRecord(ev0, st);
     // One merged rev residue per k needs every j digit group's exponent
     // a-c-lj >= 0; otherwise track the groups separately.
@@ -625,6 +636,7 @@ void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long
         td_large_kernel<DJ, false><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
             d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits);
     if (ev1) (void)hipEventRecord(ev1, st);
+    (void)hipStreamWaitEvent(st, s.join, 0);
 }
 
 int td_launch(const uint32_t *d_primes, uint32_t n, long long a, long long b, long long c,

@@ -112,3 +112,12 @@ def test_scheduler_rejects_bad_arguments_without_gpu(pf):
     assert not L.mr_sched_create(None, 0, 0, 0, 0, 0)
     assert L.mr_sched_pump(None, 0) == -1
     assert L.mr_sched_submit(None, None, None, 0) == -1
+
+
+def test_cpp_runner_header_compiles():
+    """include/batch_mod_ctx.hpp (the reference's C++ runner surface) is
+    self-contained C++17."""
+    src = "#include \"batch_mod_ctx.hpp\"\nint main() { return DEFAULT_WITNESSES.size() == 16 ? 0 : 1; }\n"
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", f"-I{INCLUDE}",
+                        "-x", "c++", "-"], input=src, text=True, capture_output=True)
+    assert r.returncode == 0, r.stderr

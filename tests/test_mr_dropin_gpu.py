@@ -67,3 +67,28 @@ def test_codes_at_1000_digits(gpu):
 def test_invalid_candidates_fail_loudly(gpu):
     assert gpu.miller_rabin_test(4, 2, 1, 0, 1)[0] == -1        # k < 1
     assert gpu.miller_rabin_test(3, 5, 1, 1, 1)[0] == -1        # N < 0
+
+
+def _limbs16(n, n_limbs):
+    return [(n >> (16 * i)) & 0xFFFF for i in range(n_limbs)]
+
+
+def test_cpp_runner_surface(gpu):
+    """batch_mod_ctx.hpp driven like bench_mr_gpu.cu:60-64 (16-bit limbs in
+    uint64, DEFAULT_WITNESSES): verdicts equal GMP's per-witness rounds."""
+    import subprocess
+    from _support import PKG, gmp_sprp
+    exe = PKG / "lib" / "mr_cpp_check"
+    assert exe.exists(), "mr_cpp_check not built"
+    nums = [2 ** 521 - 1, 2 ** 607 - 1, 2047 * (2 ** 400 + 1), 3215031751,
+            gmp_build(1000, 500, 100, 32, 5, 0), gmp_build(1000, 500, 100, 1, 7, 0),
+            gmp_build(1000, 500, 100, 54, 26, 2)]
+    n_limbs = max((n.bit_length() + 15) // 16 for n in nums)
+    text = f"{n_limbs} {len(nums)}\n" + " ".join(str(x) for n in nums for x in _limbs16(n, n_limbs))
+    r = subprocess.run([str(exe)], input=text, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = [int(l.split()[1]) for l in r.stdout.splitlines() if l and l[0].isdigit()]
+    witnesses = [2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53]
+    exp = [int(all(gmp_sprp(n, w)[0] for w in witnesses)) for n in nums]
+    assert got == exp
+    assert exp[3] == 0          # 3215031751 fails at 11
