@@ -117,14 +117,41 @@ __device__ __forceinline__ int grp_addr(int b, int j)
 // Per (prime, M): tw[i] = w^i and itw[i] = w^-i for i < M/2 with their Shoup
 // quotients, w a primitive M-th root of unity.  Layout of one table block:
 // [tw | twq | itw | itwq], each M/2 words; prime 1 follows prime 0.
+//
+// Per-pass tables (`pass`): the three twiddles (and quotients) a radix-4
+// pass needs at butterfly position pos, stored at consecutive pos, six
+// arrays of H words per pass ([a | a' | b | b' | c | c']), so that the
+// lanes of a wave, whose positions are consecutive, read consecutive words.
+// Indexing the full tables instead (w^(pos << S)) puts the lanes 2^S words
+// apart: one cache line per lane and per load.  Block of 2M words per
+// (prime, direction): forward passes at their fwd_off, inverse passes at
+// their inv_off (both below 2M).
 struct Twiddles {
     const uint32_t *base;   // 2 primes * 4 * (M/2) words
     int half;               // M/2
+    const uint32_t *pass;   // 2 primes * 2 directions * 2M words
     __device__ __forceinline__ const uint32_t *tw(int q) const { return base + (size_t)q * 4 * half; }
     __device__ __forceinline__ const uint32_t *twq(int q) const { return tw(q) + half; }
     __device__ __forceinline__ const uint32_t *itw(int q) const { return tw(q) + 2 * half; }
     __device__ __forceinline__ const uint32_t *itwq(int q) const { return tw(q) + 3 * half; }
+    __device__ __forceinline__ const uint32_t *fwd(int q) const { return pass + (size_t)(2 * q) * 4 * half; }
+    __device__ __forceinline__ const uint32_t *inv(int q) const { return pass + (size_t)(2 * q + 1) * 4 * half; }
 };
+
+// Word offsets of the per-pass blocks (same formulas on the host).
+__host__ __device__ constexpr int fwd_off(int logm, int S)
+{
+    int o = 0;
+    for (int s2 = 0; s2 < S; s2 += 2) o += 6 * ((1 << logm) >> (s2 + 2));
+    return o;
+}
+__host__ __device__ constexpr int inv_first(int logm) { return (logm & 1) ? 1 : 2; }
+__host__ __device__ constexpr int inv_off(int logm, int T0)
+{
+    int o = 0;
+    for (int t = inv_first(logm); t < T0; t += 2) o += 6 * (1 << t);
+    return o;
+}
 
 // ------------------------------------------------------------- transforms
 // ------------------------------------------------- fused transform passes
@@ -164,16 +191,14 @@ __device__ constexpr bool dit_hoisted() { return (1 << T0) <= THREADS; }
 template <int LOGM, int S, int NP>
 __device__ __forceinline__ TwSet<NP> dif_tw(const Twiddles &T, int q0)
 {
-    constexpr int H2 = (1 << LOGM) >> (S + 2);
+    constexpr int H2 = (1 << LOGM) >> (S + 2), OFF = fwd_off(LOGM, S);
     const int pos = tid_() & (H2 - 1);
-    const int ia = pos << S, ib = (pos + H2) << S, ic = pos << (S + 1);
     TwSet<NP> t;
 #pragma unroll
     for (int q = 0; q < NP; q++) {
-        const uint32_t *tw = T.tw(q0 + q), *twq = T.twq(q0 + q);
-        t.v[q][0] = tw[ia]; t.v[q][1] = twq[ia];
-        t.v[q][2] = tw[ib]; t.v[q][3] = twq[ib];
-        t.v[q][4] = tw[ic]; t.v[q][5] = twq[ic];
+        const uint32_t *b = T.fwd(q0 + q) + OFF + pos;
+#pragma unroll
+        for (int k = 0; k < 6; k++) t.v[q][k] = b[k * H2];
     }
     return t;
 }
@@ -181,16 +206,14 @@ __device__ __forceinline__ TwSet<NP> dif_tw(const Twiddles &T, int q0)
 template <int LOGM, int T0, int NP>
 __device__ __forceinline__ TwSet<NP> dit_tw(const Twiddles &T, int q0)
 {
-    constexpr int H = 1 << T0;
+    constexpr int H = 1 << T0, OFF = inv_off(LOGM, T0);
     const int pos = tid_() & (H - 1);
-    const int ia = pos << (LOGM - 1 - T0), ib = pos << (LOGM - 2 - T0), ic = (pos + H) << (LOGM - 2 - T0);
     TwSet<NP> t;
 #pragma unroll
     for (int q = 0; q < NP; q++) {
-        const uint32_t *itw = T.itw(q0 + q), *itwq = T.itwq(q0 + q);
-        t.v[q][0] = itw[ia]; t.v[q][1] = itwq[ia];
-        t.v[q][2] = itw[ib]; t.v[q][3] = itwq[ib];
-        t.v[q][4] = itw[ic]; t.v[q][5] = itwq[ic];
+        const uint32_t *b = T.inv(q0 + q) + OFF + pos;
+#pragma unroll
+        for (int k = 0; k < 6; k++) t.v[q][k] = b[k * H];
     }
     return t;
 }
@@ -261,7 +284,7 @@ __device__ __forceinline__ void dif_pass(uint32_t *X, const Twiddles &T, int q0,
             for (int q = 0; q < NP; q++) {
                 uint32_t *Y = X + q * MP;
                 const uint32_t p = prime(q0 + q);
-                const uint32_t *tw = T.tw(q0 + q), *twq = T.twq(q0 + q);
+                const uint32_t *tp = T.fwd(q0 + q) + fwd_off(LOGM, S);
                 uint32_t wa = 0, waq = 0, wb = 0, wbq = 0, wc = 0, wcq = 0;
                 if constexpr (H2 <= THREADS) {
                     wa = cur.v[q][0]; waq = cur.v[q][1]; wb = cur.v[q][2];
@@ -271,8 +294,8 @@ __device__ __forceinline__ void dif_pass(uint32_t *X, const Twiddles &T, int q0,
                 for (int ib = 0; ib < GB; ib++) {
                     if constexpr (H2 > THREADS) {
                         const int pos = pass_group<H2, GROUPS>(tid + (i0 + ib) * THREADS) & (H2 - 1);
-                        const int ia = pos << S, ibw = (pos + H2) << S, ic = pos << (S + 1);
-                        wa = tw[ia]; waq = twq[ia]; wb = tw[ibw]; wbq = twq[ibw]; wc = tw[ic]; wcq = twq[ic];
+                        const uint32_t *tb = tp + pos;
+                        wa = tb[0]; waq = tb[H2]; wb = tb[2 * H2]; wbq = tb[3 * H2]; wc = tb[4 * H2]; wcq = tb[5 * H2];
                     }
                     const uint32_t x0 = x[q][ib][0], x1 = x[q][ib][1], x2 = x[q][ib][2], x3 = x[q][ib][3];
                     const uint32_t y0 = add_mod(x0, x2, p);
@@ -323,7 +346,7 @@ __device__ __forceinline__ void dit_pass(uint32_t *X, const Twiddles &T, int q0,
             for (int q = 0; q < NP; q++) {
                 uint32_t *Y = X + q * MP;
                 const uint32_t p = prime(q0 + q);
-                const uint32_t *itw = T.itw(q0 + q), *itwq = T.itwq(q0 + q);
+                const uint32_t *tp = T.inv(q0 + q) + inv_off(LOGM, T0);
                 uint32_t wa = 0, waq = 0, wb = 0, wbq = 0, wc = 0, wcq = 0;
                 if constexpr (H <= THREADS) {
                     wa = cur.v[q][0]; waq = cur.v[q][1]; wb = cur.v[q][2];
@@ -333,9 +356,8 @@ __device__ __forceinline__ void dit_pass(uint32_t *X, const Twiddles &T, int q0,
                 for (int ib = 0; ib < GB; ib++) {
                     if constexpr (H > THREADS) {
                         const int pos = pass_group<H, GROUPS>(tid + (i0 + ib) * THREADS) & (H - 1);
-                        const int ia = pos << (LOGM - 1 - T0), ibw = pos << (LOGM - 2 - T0),
-                                  ic = (pos + H) << (LOGM - 2 - T0);
-                        wa = itw[ia]; waq = itwq[ia]; wb = itw[ibw]; wbq = itwq[ibw]; wc = itw[ic]; wcq = itwq[ic];
+                        const uint32_t *tb = tp + pos;
+                        wa = tb[0]; waq = tb[H]; wb = tb[2 * H]; wbq = tb[3 * H]; wc = tb[4 * H]; wcq = tb[5 * H];
                     }
                     const uint32_t x0 = x[q][ib][0], x1 = x[q][ib][1], x2 = x[q][ib][2], x3 = x[q][ib][3];
                     const uint32_t b1 = shoup(x1, wa, waq, p);

@@ -777,13 +777,51 @@ constexpr bool G_NP1(int logm) { return logm > 14; }
 
 struct TwCache {
     uint32_t *tw[MAX_LOGM + 1] = {};
+    uint32_t *pass[MAX_LOGM + 1] = {};
 };
 TwCache g_tw[64];
 
-int get_twiddles(int dev, int logm, hipStream_t st, uint32_t **out)
+// Per-pass twiddle tables (layout: Twiddles::pass in mr_core.h), built on
+// the host: [prime][direction][2M words].
+std::vector<uint32_t> pass_tables(int logm)
+{
+    const int M = 1 << logm, LAST = (logm & 1) ? logm - 1 : logm - 2;
+    std::vector<uint32_t> h((size_t)2 * 2 * 2 * M, 0u);
+    std::vector<uint32_t> pw(M), pwi(M);
+    for (int q = 0; q < 2; q++) {
+        const uint32_t p = q ? P1 : P0;
+        const uint64_t w = h_pow(q ? G1 : G0, (p - 1) >> logm, p), wi = h_pow(w, p - 2, p);
+        pw[0] = pwi[0] = 1u;
+        for (int i = 1; i < M; i++) {
+            pw[i] = (uint32_t)(pw[i - 1] * w % p);
+            pwi[i] = (uint32_t)(pwi[i - 1] * wi % p);
+        }
+        auto put = [&](uint32_t *blk, int H, int pos, uint32_t a, uint32_t b, uint32_t c) {
+            blk[pos] = a; blk[H + pos] = h_shoupq(a, p);
+            blk[2 * H + pos] = b; blk[3 * H + pos] = h_shoupq(b, p);
+            blk[4 * H + pos] = c; blk[5 * H + pos] = h_shoupq(c, p);
+        };
+        uint32_t *fwd = h.data() + (size_t)(2 * q) * 2 * M, *inv = fwd + 2 * M;
+        for (int S = 0; S < LAST; S += 2) {
+            const int H2 = M >> (S + 2);
+            for (int pos = 0; pos < H2; pos++)
+                put(fwd + fwd_off(logm, S), H2, pos, pw[pos << S], pw[(pos + H2) << S], pw[pos << (S + 1)]);
+        }
+        for (int T0 = inv_first(logm); T0 + 1 < logm; T0 += 2) {
+            const int H = 1 << T0;
+            for (int pos = 0; pos < H; pos++)
+                put(inv + inv_off(logm, T0), H, pos, pwi[pos << (logm - 1 - T0)], pwi[pos << (logm - 2 - T0)],
+                    pwi[(pos + H) << (logm - 2 - T0)]);
+        }
+    }
+    return h;
+}
+
+int get_twiddles(int dev, int logm, hipStream_t st, uint32_t **out, uint32_t **pass_out)
 {
     if (dev < 0 || dev >= 64) return -1;
     uint32_t *&t = g_tw[dev].tw[logm];
+    uint32_t *&pt = g_tw[dev].pass[logm];
     if (!t) {
         const int half = 1 << (logm - 1);
         if (hipMalloc(&t, (size_t)2 * 4 * half * sizeof(uint32_t)) != hipSuccess) { t = nullptr; return -1; }
@@ -792,7 +830,13 @@ int get_twiddles(int dev, int logm, hipStream_t st, uint32_t **out)
         twiddle_kernel<<<(half + 255) / 256, 256, 0, st>>>(t, logm, w0, w1);
         if (hipStreamSynchronize(st) != hipSuccess) return -1;
     }
+    if (!pt) {
+        const std::vector<uint32_t> h = pass_tables(logm);
+        if (hipMalloc(&pt, h.size() * sizeof(uint32_t)) != hipSuccess) { pt = nullptr; return -1; }
+        if (hipMemcpy(pt, h.data(), h.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) return -1;
+    }
     *out = t;
+    *pass_out = pt;
     return 0;
 }
 
@@ -905,9 +949,10 @@ int ctx_alloc_all(mr_ctx *x, long bits, int min_logm)
 #undef MR_ALLOC
     d.list = nullptr;
     fill_i32_kernel<<<(x->n + 255) / 256, 256, 0, x->st>>>(d.Ls, x->n, L);
-    uint32_t *tw = nullptr;
-    if (get_twiddles(x->device, logm, x->st, &tw) != 0) return hip_fail(hipErrorUnknown, "twiddles");
+    uint32_t *tw = nullptr, *ptw = nullptr;
+    if (get_twiddles(x->device, logm, x->st, &tw, &ptw) != 0) return hip_fail(hipErrorUnknown, "twiddles");
     x->T.base = tw;
+    x->T.pass = ptw;
     x->T.half = M / 2;
     x->K = make_consts(logm);
     return 0;
