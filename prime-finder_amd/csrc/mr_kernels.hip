@@ -43,6 +43,8 @@ struct Dev {                    // per-batch device state (kernel argument)
     int32_t *bidx;              // [n] index of that base in the scheduler's list
     uint8_t *done;              // [n] scheduler: 1 = verdict final (in alive)
     uint32_t *N;                // [n][stride] modulus limbs
+    uint32_t *Nh;               // [n][M] N in the high layout (positions [L, 2L)),
+                                //   lane-interleaved: element i of thread t at i * THREADS + t
     uint32_t *Nf;               // [n][2M] NTT(N) * 2^32 per prime
     uint32_t *Npf;              // [n][2M] NTT(N') * 2^32 per prime
     uint32_t *one;              // [n][stride] R mod N
@@ -222,13 +224,15 @@ __device__ void to_fft(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
     }
 }
 
-// r <- r - N when r >= N; N read from Nl at layout offset off.  r < 2N.
+// r <- r - N when r >= N; N read from Nl at layout offset off, or (Nil
+// non-null) from its lane-interleaved copy (coalesced).  r < 2N.
 template <int C>
 __device__ __forceinline__ void reduce_once(uint32_t (&r)[C], const uint32_t *Nl, int L, int off,
-                                            Scratch &sc)
+                                            Scratch &sc, const uint32_t *Nil = nullptr)
 {
     uint32_t t[C], d[C];
-    load_vec<C>(t, Nl, L, off);
+    if (Nil) load_own<C>(t, Nil, 0, C * THREADS);
+    else load_vec<C>(t, Nl, L, off);
     const bool lt = sub_vec<C>(d, r, t, sc);
     if (!lt) {
 #pragma unroll
@@ -241,7 +245,7 @@ __device__ __forceinline__ void reduce_once(uint32_t (&r)[C], const uint32_t *Nl
 // that at most three limb vectors are live.
 template <int C>
 __device__ void mulsmall_modN(uint32_t (&r)[C], uint32_t s, const uint32_t *Nl, int L, int off,
-                              uint32_t *stash, Scratch &sc)
+                              uint32_t *stash, Scratch &sc, const uint32_t *Nil = nullptr)
 {
     const int p0 = tid_() * C;
 #pragma unroll
@@ -249,13 +253,13 @@ __device__ void mulsmall_modN(uint32_t (&r)[C], uint32_t s, const uint32_t *Nl, 
     const int top = 31 - __clz((int)s);
     for (int b = top - 1; b >= 0; b--) {
         add_vec<C>(r, r, 0u, sc);                            // 2r < 2N
-        reduce_once<C>(r, Nl, L, off, sc);
+        reduce_once<C>(r, Nl, L, off, sc, Nil);
         if ((s >> b) & 1u) {
             uint32_t t[C];
 #pragma unroll
             for (int i = 0; i < C; i++) t[i] = stash[px(p0 + i)];
             add_vec<C>(r, t, 0u, sc);
-            reduce_once<C>(r, Nl, L, off, sc);
+            reduce_once<C>(r, Nl, L, off, sc, Nil);
         }
     }
 }
@@ -302,7 +306,7 @@ __device__ void montmul(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, cons
         for (int i = 0; i < C; i++) t[i] += (cbit && p0 + i == L) ? 1u : 0u;
         add_vec<C>(r, t, 0u, sm.sc);                           // T_hi + (mN)_hi + c
     }
-    reduce_once<C>(r, Nl, L, L, sm.sc);
+    reduce_once<C>(r, Nl, L, L, sm.sc, D.Nh + (size_t)c * M);
 }
 
 // x <- x * s (plain integer product), 1 <= s < 2^24.
@@ -517,6 +521,11 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
     }
     const int nbits = (int)sm.bcast[0];
     if (D.pre[c] != 0) return;
+    {   // N in the high layout, lane-interleaved, for the reductions
+        uint32_t h[C];
+        load_vec<C>(h, Nl, L, L);
+        save_own<C>(h, D.Nh + (size_t)c * M, 0, M);
+    }
 
     uint32_t *Nf = D.Nf + (size_t)c * 2 * M;
     uint32_t *Npf = D.Npf + (size_t)c * 2 * M;
@@ -633,7 +642,7 @@ __global__ __launch_bounds__(THREADS) void exp_kernel(Dev D, Twiddles T, Consts 
     for (int k = 0; k < steps && bp >= s; k++, bp--) {
         montmul<LOGM>(sm, T, K, D, c, r, L, nullptr);
         const uint32_t bit = (Nl[bp / 24] >> (bp % 24)) & 1u;
-        if (bit) mulsmall_modN<C>(r, base, Nl, L, L, sm.X, sm.sc);
+        if (bit) mulsmall_modN<C>(r, base, Nl, L, L, sm.X, sm.sc, D.Nh + (size_t)c * G<LOGM>::M);
     }
     store_vec<C>(r, D.res + (size_t)c * D.stride, L, L);
     if (threadIdx.x == 0) D.bitpos[c] = bp;
@@ -929,6 +938,7 @@ int ctx_alloc_all(mr_ctx *x, long bits, int min_logm)
     if (ctx_alloc(x, &p, sizeof(type) * (size_t)(count)) != 0) return -2; \
     d.field = (type *)p;
     MR_ALLOC(N, uint32_t, n * x->stride);
+    MR_ALLOC(Nh, uint32_t, n * M);
     MR_ALLOC(Nf, uint32_t, n * 2 * M);
     MR_ALLOC(Npf, uint32_t, n * 2 * M);
     MR_ALLOC(one, uint32_t, n * x->stride);
@@ -1389,7 +1399,7 @@ struct Req {
 int64_t slot_bytes(int logm)
 {
     const int64_t M = 1ll << logm;
-    return 4 * (4 * M + 6 * M + (G_NP1(logm) ? M : 0)) + 32;
+    return 4 * (4 * M + 7 * M + (G_NP1(logm) ? M : 0)) + 32;
 }
 
 struct Pool {
