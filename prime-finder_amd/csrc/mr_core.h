@@ -130,12 +130,21 @@ struct Twiddles {
     const uint32_t *base;   // 2 primes * 4 * (M/2) words
     int half;               // M/2
     const uint32_t *pass;   // 2 primes * 2 directions * 2M words
+    // Products of 2^15 points run as two transforms of 2^14 (base, half and
+    // pass are then the 2^14 tables); big holds the 2^15 table w^i, w^-i
+    // (i < 2^14, with quotients), big_half = 2^14, for the half twist.
+    const uint32_t *big;
+    int big_half;
     __device__ __forceinline__ const uint32_t *tw(int q) const { return base + (size_t)q * 4 * half; }
     __device__ __forceinline__ const uint32_t *twq(int q) const { return tw(q) + half; }
     __device__ __forceinline__ const uint32_t *itw(int q) const { return tw(q) + 2 * half; }
     __device__ __forceinline__ const uint32_t *itwq(int q) const { return tw(q) + 3 * half; }
     __device__ __forceinline__ const uint32_t *fwd(int q) const { return pass + (size_t)(2 * q) * 4 * half; }
     __device__ __forceinline__ const uint32_t *inv(int q) const { return pass + (size_t)(2 * q + 1) * 4 * half; }
+    __device__ __forceinline__ const uint32_t *btw(int q) const { return big + (size_t)q * 4 * big_half; }
+    __device__ __forceinline__ const uint32_t *btwq(int q) const { return btw(q) + big_half; }
+    __device__ __forceinline__ const uint32_t *bitw(int q) const { return btw(q) + 2 * big_half; }
+    __device__ __forceinline__ const uint32_t *bitwq(int q) const { return btw(q) + 3 * big_half; }
 };
 
 // Word offsets of the per-pass blocks (same formulas on the host).
@@ -412,7 +421,7 @@ __device__ __forceinline__ void dit_regular(uint32_t *X, const Twiddles &T, int 
 // times B (kept times 2^32) -- one code path, a uniform branch.
 template <int LOGM, int NP, int MODE>
 __device__ __forceinline__ void middle(uint32_t *X, const Twiddles &T, int q0, const uint32_t *B,
-                                       const uint32_t *np_tab)
+                                       const uint32_t *np_tab, int bstride)
 {
     constexpr int M = 1 << LOGM, MP = Lds<LOGM>::MP;
     const int tid = tid_();
@@ -420,7 +429,7 @@ __device__ __forceinline__ void middle(uint32_t *X, const Twiddles &T, int q0, c
     for (int q = 0; q < NP; q++) {
         uint32_t *Y = X + q * MP;
         const uint32_t p = prime(q0 + q), np = np_tab[q0 + q];
-        const uint32_t *Bq = B ? B + (size_t)(q0 + q) * M : nullptr;
+        const uint32_t *Bq = B ? B + (size_t)(q0 + q) * bstride : nullptr;
         if constexpr ((LOGM & 1) == 0) {
             constexpr int GROUPS = M / 4;
             constexpr int IT = GROUPS >= THREADS ? GROUPS / THREADS : 1;
@@ -486,17 +495,18 @@ __device__ __forceinline__ void middle(uint32_t *X, const Twiddles &T, int q0, c
 }
 
 // Cyclic convolution of the NP buffers with themselves (B null) or with the
-// stored transform B; output scaled by M (and by 2^-32 for the square).
+// stored transform B (prime q's at B + q * bstride); output scaled by M (and
+// by 2^-32 for the square).
 template <int LOGM, int NP>
 __device__ __forceinline__ void convolve(uint32_t *X, const Twiddles &T, int q0, const uint32_t *B,
-                                         const uint32_t *np_tab)
+                                         const uint32_t *np_tab, int bstride = 1 << LOGM)
 {
     constexpr int T0 = (LOGM & 1) ? 1 : 2;     // first inverse pass after middle
     TwSet<NP> f{}, d{};
     if constexpr (dif_hoisted<LOGM, 0>()) f = dif_tw<LOGM, 0, NP>(T, q0);
     dif_regular<LOGM, 0, NP>(X, T, q0, f);
     if constexpr (T0 + 1 < LOGM && dit_hoisted<LOGM, T0>()) d = dit_tw<LOGM, T0, NP>(T, q0);
-    middle<LOGM, NP, 1>(X, T, q0, B, np_tab);
+    middle<LOGM, NP, 1>(X, T, q0, B, np_tab, bstride);
     dit_regular<LOGM, T0, NP>(X, T, q0, d);
 }
 
@@ -507,7 +517,7 @@ __device__ __forceinline__ void forward(uint32_t *X, const Twiddles &T, int q0, 
     TwSet<NP> f{};
     if constexpr (dif_hoisted<LOGM, 0>()) f = dif_tw<LOGM, 0, NP>(T, q0);
     dif_regular<LOGM, 0, NP>(X, T, q0, f);
-    middle<LOGM, NP, 0>(X, T, q0, nullptr, np_tab);
+    middle<LOGM, NP, 0>(X, T, q0, nullptr, np_tab, 0);
 }
 
 // ------------------------------------------------------------- scans
@@ -572,17 +582,25 @@ __device__ __forceinline__ void crt_carry(uint32_t (&v)[C], Get0 get0, Get1 get1
 {
     const int t = tid_();
     uint64_t acc = 0;
+    // inputs in batches of CB, all of a batch's reads issued before use
+    // (some callers read them from global memory)
+    constexpr int CB = C < 8 ? C : 8;
 #pragma unroll
-    for (int i = 0; i < C; i++) {
-        const uint32_t y0 = get0(i), y1 = get1(i);
-        const uint32_t a0 = shoup(y0, k.s0, k.s0q, P0);
-        const uint32_t u = shoup(y1, k.c1, k.c1q, P1);
-        const uint32_t w = shoup(a0, k.c2, k.c2q, P1);
-        const uint32_t tt = sub_mod(u, w, P1);
-        const uint64_t x = (uint64_t)tt * P0 + a0;
-        acc += x;
-        v[i] = (uint32_t)acc & LIMB_MASK;
-        acc >>= LIMB_BITS;
+    for (int i0 = 0; i0 < C; i0 += CB) {
+        uint32_t y0[CB], y1[CB];
+#pragma unroll
+        for (int i = 0; i < CB; i++) { y0[i] = get0(i0 + i); y1[i] = get1(i0 + i); }
+#pragma unroll
+        for (int i = 0; i < CB; i++) {
+            const uint32_t a0 = shoup(y0[i], k.s0, k.s0q, P0);
+            const uint32_t u = shoup(y1[i], k.c1, k.c1q, P1);
+            const uint32_t w = shoup(a0, k.c2, k.c2q, P1);
+            const uint32_t tt = sub_mod(u, w, P1);
+            const uint64_t x = (uint64_t)tt * P0 + a0;
+            acc += x;
+            v[i0 + i] = (uint32_t)acc & LIMB_MASK;
+            acc >>= LIMB_BITS;
+        }
     }
     sc.carry[t] = acc;               // < 2^40
     __syncthreads();

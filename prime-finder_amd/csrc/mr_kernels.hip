@@ -51,7 +51,7 @@ struct Dev {                    // per-batch device state (kernel argument)
     uint32_t *nm1;              // [n][stride] (N - 1) R mod N
     uint32_t *res;              // [n][stride] current residue (Montgomery form)
     uint32_t *scr;              // [n][2M] transform scratch
-    uint32_t *ysv;              // [n][M] at M = 2^15: first prime's product while the second runs
+    uint32_t *ysv;              // [n][M + M/2] for split products: first half's inverse, staged operand
     int32_t *nbits;             // [n]
     int32_t *sval;              // [n] s with N - 1 = 2^s d
     int32_t *bitpos;            // [n] next exponent bit (bit index of N), < s when done
@@ -71,16 +71,28 @@ struct Consts {
     uint32_t np[2];             // -p^-1 mod 2^32
 };
 
+// Geometry of a product of M = 2^LOGM points.  Up to 2^14 one transform of
+// M points per prime; at 2^15 (SPLIT) two transforms of 2^14 per prime: the
+// operands' upper halves are zero, so the 2^15 transform's first stage
+// splits it into the transforms of a and of a * w^n (w a 2^15-th root of
+// unity), which give the even and the odd frequencies (the two halves of the
+// bit-reversed output); the inverse's last stage recombines
+// c[n] = u[n] + w^-n v[n], c[n + 2^14] = u[n] - w^-n v[n].  Both primes'
+// buffers of TL = 14 fit the LDS at once either way.
 template <int LOGM>
 struct G {
     static constexpr int M = 1 << LOGM;
     static constexpr int C = M / THREADS;
-    static constexpr int NP = LOGM <= 14 ? 2 : 1;
+    static constexpr int NP = 2;
+    static constexpr bool SPLIT = LOGM > 14;
+    static constexpr int TL = SPLIT ? 14 : LOGM;      // log2 of each transform
+    static constexpr int H = 1 << TL;
+    static constexpr int YSV = SPLIT ? M + H : 0;      // per-slot words of Dev::ysv
 };
 
 template <int LOGM>
 struct Smem {
-    uint32_t X[G<LOGM>::NP * Lds<LOGM>::MP];      // padded transform buffers
+    uint32_t X[2 * Lds<G<LOGM>::TL>::MP];           // padded transform buffers, one per prime
     Scratch sc;
     uint32_t bcast[8];
 };
@@ -142,16 +154,20 @@ __device__ __forceinline__ void zero_below(uint32_t (&v)[C], int lim)
 
 // Put positions [off, off+len) of v at X_q[0..len), zero elsewhere, for the
 // NPW buffers starting at X.
+// Only positions below H = 2^TL exist in the buffers (the operand has at
+// most L < H limbs).
 template <int LOGM, int NPW>
 __device__ __forceinline__ void stage_in(uint32_t *X, const uint32_t (&v)[G<LOGM>::C], int off, int len)
 {
-    constexpr int C = G<LOGM>::C, MP = Lds<LOGM>::MP;
+    constexpr int C = G<LOGM>::C, MP = Lds<G<LOGM>::TL>::MP, H = G<LOGM>::H;
     const int p0 = tid_() * C;
     if (off != 0) {
+        if (p0 < H) {
 #pragma unroll
-        for (int q = 0; q < NPW; q++)
+            for (int q = 0; q < NPW; q++)
 #pragma unroll
-            for (int i = 0; i < C; i++) X[q * MP + px(p0 + i)] = 0u;
+                for (int i = 0; i < C; i++) X[q * MP + px(p0 + i)] = 0u;
+        }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < C; i++) {
@@ -160,7 +176,7 @@ __device__ __forceinline__ void stage_in(uint32_t *X, const uint32_t (&v)[G<LOGM
 #pragma unroll
                 for (int q = 0; q < NPW; q++) X[q * MP + px(idx)] = v[i];
         }
-    } else {
+    } else if (p0 < H) {
 #pragma unroll
         for (int i = 0; i < C; i++) {
             const uint32_t x = (p0 + i < len) ? v[i] : 0u;
@@ -171,54 +187,142 @@ __device__ __forceinline__ void stage_in(uint32_t *X, const uint32_t (&v)[G<LOGM
     __syncthreads();
 }
 
+// Split products: the staged operand (natural order, prime 0's buffer) is
+// copied to save (H words, coalesced) ...
+template <int LOGM>
+__device__ __forceinline__ void split_save(const uint32_t *X, uint32_t *save)
+{
+    constexpr int H = G<LOGM>::H, KN = H / THREADS;
+    const int t = tid_();
+#pragma unroll
+    for (int k = 0; k < KN; k++) save[t + k * THREADS] = X[px(t + k * THREADS)];
+    __syncthreads();
+}
+
+// ... and restaged for the second half as a[n] * w^n in both buffers.  The
+// loops have compile-time trip counts and issue a batch's global loads
+// before using any, so that their latency is paid once per batch.
+template <int LOGM>
+__device__ __forceinline__ void split_twist(uint32_t *X, const Twiddles &T, const uint32_t *save)
+{
+    constexpr int H = G<LOGM>::H, MP = Lds<G<LOGM>::TL>::MP, KN = H / THREADS, KB = 8;
+    const int t = tid_();
+    const uint32_t *w0 = T.btw(0), *q0 = T.btwq(0), *w1 = T.btw(1), *q1 = T.btwq(1);
+#pragma unroll
+    for (int k0 = 0; k0 < KN; k0 += KB) {
+        uint32_t x[KB], a0[KB], b0[KB], a1[KB], b1[KB];
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+            const int n = t + (k0 + k) * THREADS;
+            x[k] = save[n]; a0[k] = w0[n]; b0[k] = q0[n]; a1[k] = w1[n]; b1[k] = q1[n];
+        }
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+            const int n = t + (k0 + k) * THREADS;
+            X[px(n)] = shoup(x[k], a0[k], b0[k], P0);
+            X[MP + px(n)] = shoup(x[k], a1[k], b1[k], P1);
+        }
+    }
+    __syncthreads();
+}
+
+// Inverse of the second half, times w^-n (the recombination's weight).
+template <int LOGM>
+__device__ __forceinline__ void split_unweight(uint32_t *X, const Twiddles &T)
+{
+    constexpr int H = G<LOGM>::H, MP = Lds<G<LOGM>::TL>::MP, KN = H / THREADS, KB = 8;
+    const int t = tid_();
+    const uint32_t *w0 = T.bitw(0), *q0 = T.bitwq(0), *w1 = T.bitw(1), *q1 = T.bitwq(1);
+#pragma unroll
+    for (int k0 = 0; k0 < KN; k0 += KB) {
+        uint32_t a0[KB], b0[KB], a1[KB], b1[KB];
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+            const int n = t + (k0 + k) * THREADS;
+            a0[k] = w0[n]; b0[k] = q0[n]; a1[k] = w1[n]; b1[k] = q1[n];
+        }
+#pragma unroll
+        for (int k = 0; k < KB; k++) {
+            const int n = t + (k0 + k) * THREADS;
+            X[px(n)] = shoup(X[px(n)], a0[k], b0[k], P0);
+            X[MP + px(n)] = shoup(X[MP + px(n)], a1[k], b1[k], P1);
+        }
+    }
+    lds_barrier();
+}
+
 // out = A * B as normalised limbs (positions >= keep zeroed).  A is
 // positions [off, off+len) of a; B is a (square) when Bf == nullptr, else the
-// precomputed transform Bf.  a and out may alias.  With one prime per LDS
-// pass (M = 2^15) the first prime's result waits in ysave (M words).
+// precomputed transform Bf.  a and out may alias.  Split products (M = 2^15)
+// park the first half's inverse u in ysave (M words, lane-interleaved by
+// the reading thread) and the staged operand at ysave + M (H words).
 template <int LOGM>
 __device__ void tproduct(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
                          const uint32_t (&a)[G<LOGM>::C], int off, int len,
                          const uint32_t *Bf, uint32_t (&out)[G<LOGM>::C], int keep,
                          uint32_t *ysave)
 {
-    constexpr int C = G<LOGM>::C, NP = G<LOGM>::NP;
+    constexpr int C = G<LOGM>::C, M = G<LOGM>::M, TL = G<LOGM>::TL, MP = Lds<TL>::MP;
     const CrtConst k = Bf ? K.mul : K.sq;
     const int p0 = tid_() * C;
-    if constexpr (NP == 2) {
-        stage_in<LOGM, 2>(sm.X, a, off, len);
+    const uint32_t *X0 = sm.X, *X1 = sm.X + MP;
+    stage_in<LOGM, 2>(sm.X, a, off, len);
+    if constexpr (!G<LOGM>::SPLIT) {
         convolve<LOGM, 2>(sm.X, T, 0, Bf, K.np);
-        const uint32_t *X0 = sm.X, *X1 = sm.X + Lds<LOGM>::MP;
         crt_carry<C>(out, [&](int i) { return X0[px(p0 + i)]; }, [&](int i) { return X1[px(p0 + i)]; },
                      k, keep, sm.sc);
     } else {
-        // One prime at a time; the first prime's product waits in ysave.
-        stage_in<LOGM, 1>(sm.X, a, off, len);
-        convolve<LOGM, 1>(sm.X, T, 0, Bf, K.np);
+        constexpr int H = G<LOGM>::H, HALF_T = THREADS / 2;   // threads covering [0, H)
+        static_assert(C * HALF_T == H, "split layout");
+        uint32_t *asave = ysave + M;
+        split_save<LOGM>(sm.X, asave);
+        convolve<TL, 2>(sm.X, T, 0, Bf, K.np, M);                      // even frequencies
+        {   // u -> ysave: element i of position block t' (n = t' C + i) at
+            // q H + i HALF_T + t'; thread t writes half of t mod HALF_T's run
+            const int t = tid_(), tp = t & (HALF_T - 1), i0 = (t / HALF_T) * (C / 2);
 #pragma unroll
-        for (int i = 0; i < C; i++) ysave[i * THREADS + tid_()] = sm.X[px(p0 + i)];   // coalesced
+            for (int i = 0; i < C / 2; i++) {
+                ysave[(i0 + i) * HALF_T + tp] = X0[px(tp * C + i0 + i)];
+                ysave[H + (i0 + i) * HALF_T + tp] = X1[px(tp * C + i0 + i)];
+            }
+        }
         __syncthreads();
-        stage_in<LOGM, 1>(sm.X, a, off, len);
-        convolve<LOGM, 1>(sm.X, T, 1, Bf, K.np);
-        const uint32_t *X1 = sm.X;
-        crt_carry<C>(out, [&](int i) { return ysave[i * THREADS + tid_()]; }, [&](int i) { return X1[px(p0 + i)]; },
+        split_twist<LOGM>(sm.X, T, asave);
+        convolve<TL, 2>(sm.X, T, 0, Bf ? Bf + H : nullptr, K.np, M);   // odd frequencies
+        split_unweight<LOGM>(sm.X, T);
+        const int tp = tid_() & (HALF_T - 1);
+        const bool hi = p0 >= H;
+        crt_carry<C>(out,
+                     [&](int i) {
+                         const uint32_t u = ysave[i * HALF_T + tp], v = X0[px(tp * C + i)];
+                         return hi ? sub_mod(u, v, P0) : add_mod(u, v, P0);
+                     },
+                     [&](int i) {
+                         const uint32_t u = ysave[H + i * HALF_T + tp], v = X1[px(tp * C + i)];
+                         return hi ? sub_mod(u, v, P1) : add_mod(u, v, P1);
+                     },
                      k, keep, sm.sc);
     }
 }
 
-// dst[0..2M) = transform of positions [0, len) of v, times 2^32, per prime.
+// dst[0..2M) = transform of positions [0, len) of v, times 2^32, per prime
+// (split products: the two halves' transforms side by side).  save: H
+// scratch words for split products.
 template <int LOGM>
 __device__ void to_fft(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
-                       const uint32_t (&v)[G<LOGM>::C], int len, uint32_t *dst)
+                       const uint32_t (&v)[G<LOGM>::C], int len, uint32_t *dst, uint32_t *save)
 {
-    constexpr int M = G<LOGM>::M, NP = G<LOGM>::NP;
-    for (int q0 = 0; q0 < 2; q0 += NP) {
-        stage_in<LOGM, NP>(sm.X, v, 0, len);
-        forward<LOGM, NP>(sm.X, T, q0, K.np);
+    constexpr int M = G<LOGM>::M, TL = G<LOGM>::TL, H = G<LOGM>::H, MP = Lds<TL>::MP;
+    stage_in<LOGM, 2>(sm.X, v, 0, len);
+    if constexpr (G<LOGM>::SPLIT) split_save<LOGM>(sm.X, save);
+    for (int h = 0; h < (G<LOGM>::SPLIT ? 2 : 1); h++) {
+        if (h == 1) split_twist<LOGM>(sm.X, T, save);
+        forward<TL, 2>(sm.X, T, 0, K.np);
 #pragma unroll
-        for (int q = 0; q < NP; q++) {
-            const uint32_t p = prime(q0 + q), np = K.np[q0 + q], r2 = K.r2[q0 + q];
-            for (int i = tid_(); i < M; i += THREADS)
-                dst[(size_t)(q0 + q) * M + i] = mont(sm.X[q * Lds<LOGM>::MP + px(i)], r2, p, np);
+        for (int q = 0; q < 2; q++) {
+            const uint32_t p = prime(q), np = K.np[q], r2 = K.r2[q];
+            for (int i = tid_(); i < H; i += THREADS)
+                dst[(size_t)q * M + h * H + i] = mont(sm.X[q * MP + px(i)], r2, p, np);
         }
         __syncthreads();
     }
@@ -276,7 +380,7 @@ __device__ void montmul(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, cons
     const int L = D.Ls[c];
     const int p0 = tid_() * C;
     uint32_t *park = D.scr + (size_t)c * 2 * M + M;            // 2nd half of the scratch
-    uint32_t *ysave = D.ysv + (G<LOGM>::NP == 1 ? (size_t)c * M : 0);
+    uint32_t *ysave = D.ysv + (size_t)c * G<LOGM>::YSV;
     const uint32_t *Nl = D.N + (size_t)c * D.stride;
     const uint32_t *Npf = D.Npf + (size_t)c * 2 * M;
     const uint32_t *Nf = D.Nf + (size_t)c * 2 * M;
@@ -398,7 +502,7 @@ __global__ __launch_bounds__(THREADS) void build_kernel(Dev D, Twiddles T, Const
     for (int t = 0; t < tmax; t++) {
         const int64_t coef = tc[2 * t], ex = tc[2 * t + 1];
         if (coef == 0) continue;
-        pow5<LOGM>(sm, T, K, x, (long)ex, D.ysv + (G<LOGM>::NP == 1 ? (size_t)c * G<LOGM>::M : 0));
+        pow5<LOGM>(sm, T, K, x, (long)ex, D.ysv + (size_t)c * G<LOGM>::YSV);
         const uint32_t mag = (uint32_t)(coef < 0 ? -coef : coef);
         if (mag != 1u) mul_small_vec<C>(x, mag, sm.sc);
         shl_vec<C>(x, (long)ex, sm.X, G<LOGM>::M);
@@ -530,8 +634,9 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
     uint32_t *Nf = D.Nf + (size_t)c * 2 * M;
     uint32_t *Npf = D.Npf + (size_t)c * 2 * M;
     uint32_t *scr = D.scr + (size_t)c * 2 * M;
-    uint32_t *ysv = D.ysv + (G<LOGM>::NP == 1 ? (size_t)c * M : 0);
-    to_fft<LOGM>(sm, T, K, nv, L, Nf);
+    uint32_t *ysv = D.ysv + (size_t)c * G<LOGM>::YSV;
+    uint32_t *fsave = ysv + M;                       // split transforms' scratch
+    to_fft<LOGM>(sm, T, K, nv, L, Nf, fsave);
 
     // N' = -N^-1 mod R by Newton: y <- y (2 + N y), doubling precision.
     {
@@ -552,11 +657,11 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
 #pragma unroll
                 for (int i = 0; i < C; i++) if (p0 + i >= L) e[i] = 0u;
             }
-            to_fft<LOGM>(sm, T, K, e, L, scr);
+            to_fft<LOGM>(sm, T, K, e, L, scr, fsave);
             tproduct<LOGM>(sm, T, K, y, 0, L, scr, y, L, ysv);    // y = y (2 + N y) mod R
         }
     }
-    to_fft<LOGM>(sm, T, K, y, L, Npf);
+    to_fft<LOGM>(sm, T, K, y, L, Npf, fsave);
 
     // R mod N: 2^nbits - N, then doubled (24 L - nbits) times mod N.
     {
@@ -693,7 +798,7 @@ __global__ __launch_bounds__(THREADS) void round_end_kernel(Dev D, Twiddles T, C
             uint32_t unit[C];
 #pragma unroll
             for (int i = 0; i < C; i++) unit[i] = (tid_() == 0 && i == 0) ? 1u : 0u;
-            to_fft<LOGM>(sm, T, K, unit, L, one_f);
+            to_fft<LOGM>(sm, T, K, unit, L, one_f, D.ysv + (size_t)c * G<LOGM>::YSV + G<LOGM>::M);
         }
         montmul<LOGM>(sm, T, K, D, c, r, L, one_f);
         store_vec<C>(r, out + (size_t)c * out_stride, L, L);
@@ -723,13 +828,14 @@ __global__ __launch_bounds__(THREADS) void debug_kernel(Dev D, Twiddles T, Const
     load_vec<C>(a, A + (size_t)c * stride2, L, 0);
     load_vec<C>(b, B + (size_t)c * stride2, L, 0);
     uint32_t *scr = D.scr + (size_t)c * 2 * M;
+    uint32_t *fsave = D.ysv + (size_t)c * G<LOGM>::YSV + M;
     if (op == 0) {
-        to_fft<LOGM>(sm, T, K, b, L, scr);
+        to_fft<LOGM>(sm, T, K, b, L, scr, fsave);
         tproduct<LOGM>(sm, T, K, a, 0, L, scr, a, M,
-                       D.ysv + (G<LOGM>::NP == 1 ? (size_t)c * M : 0));
+                       D.ysv + (size_t)c * G<LOGM>::YSV);
         store_vec<C>(a, out + (size_t)c * stride2, 2 * L, 0);
     } else if (op == 1) {
-        to_fft<LOGM>(sm, T, K, b, L, scr);
+        to_fft<LOGM>(sm, T, K, b, L, scr, fsave);
         montmul<LOGM>(sm, T, K, D, c, a, 0, scr);
         store_vec<C>(a, out + (size_t)c * stride2, L, L);
     } else {
@@ -782,7 +888,7 @@ Consts make_consts(int logm)
 
 constexpr int MIN_LOGM = 11, MAX_LOGM = 15;
 int g_min_logm = MIN_LOGM;      // floor for new contexts (mr_set_min_log_len)
-constexpr bool G_NP1(int logm) { return logm > 14; }
+constexpr bool split_logm(int logm) { return logm > 14; }
 
 struct TwCache {
     uint32_t *tw[MAX_LOGM + 1] = {};
@@ -945,7 +1051,7 @@ int ctx_alloc_all(mr_ctx *x, long bits, int min_logm)
     MR_ALLOC(nm1, uint32_t, n * x->stride);
     MR_ALLOC(res, uint32_t, n * x->stride);
     MR_ALLOC(scr, uint32_t, n * 2 * M);
-    MR_ALLOC(ysv, uint32_t, G_NP1(logm) ? n * M : 1);
+    MR_ALLOC(ysv, uint32_t, split_logm(logm) ? n * (M + M / 2) : 1);
     MR_ALLOC(nbits, int32_t, n);
     MR_ALLOC(sval, int32_t, n);
     MR_ALLOC(bitpos, int32_t, n);
@@ -959,11 +1065,18 @@ int ctx_alloc_all(mr_ctx *x, long bits, int min_logm)
 #undef MR_ALLOC
     d.list = nullptr;
     fill_i32_kernel<<<(x->n + 255) / 256, 256, 0, x->st>>>(d.Ls, x->n, L);
-    uint32_t *tw = nullptr, *ptw = nullptr;
-    if (get_twiddles(x->device, logm, x->st, &tw, &ptw) != 0) return hip_fail(hipErrorUnknown, "twiddles");
+    // split products (2^15) transform halves of 2^14 and twist with the
+    // 2^15 table
+    const int tl = split_logm(logm) ? 14 : logm;
+    uint32_t *tw = nullptr, *ptw = nullptr, *btw = nullptr, *bptw = nullptr;
+    if (get_twiddles(x->device, tl, x->st, &tw, &ptw) != 0 ||
+        get_twiddles(x->device, logm, x->st, &btw, &bptw) != 0)
+        return hip_fail(hipErrorUnknown, "twiddles");
     x->T.base = tw;
     x->T.pass = ptw;
-    x->T.half = M / 2;
+    x->T.big = btw;
+    x->T.big_half = M / 2;
+    x->T.half = (1 << tl) / 2;
     x->K = make_consts(logm);
     return 0;
 }
@@ -1399,7 +1512,7 @@ struct Req {
 int64_t slot_bytes(int logm)
 {
     const int64_t M = 1ll << logm;
-    return 4 * (4 * M + 7 * M + (G_NP1(logm) ? M : 0)) + 32;
+    return 4 * (4 * M + 7 * M + (split_logm(logm) ? M + M / 2 : 0)) + 32;
 }
 
 struct Pool {
