@@ -521,6 +521,8 @@ __device__ __forceinline__ void forward(uint32_t *X, const Twiddles &T, int q0, 
 }
 
 // ------------------------------------------------------------- scans
+// (their barriers hand off LDS only: lds_barrier, which does not wait for
+// the wave's outstanding global stores)
 struct Scratch {            // LDS scratch beside the transform buffers
     uint64_t carry[THREADS];
     uint32_t wave[2 * WAVES + 8];
@@ -541,10 +543,10 @@ __device__ __forceinline__ uint32_t block_carry_in(bool g, bool pr, Scratch &sc)
         sc.wave[2 * w] = (s0 < G) ? 1u : 0u;          // carry out of the wave, carry-in 0
         sc.wave[2 * w + 1] = (P == ~0ull) ? 1u : 0u;   // a carry-in passes straight through
     }
-    __syncthreads();
+    lds_barrier();
     uint32_t cin = 0;
     for (int i = 0; i < w; i++) cin = sc.wave[2 * i] | (sc.wave[2 * i + 1] & cin);
-    __syncthreads();
+    lds_barrier();
     const uint64_t s = G + Y + cin;
     return (uint32_t)(((s ^ G ^ Y) >> lane) & 1u);
 }
@@ -554,12 +556,12 @@ __device__ __forceinline__ bool block_any(bool v, Scratch &sc)
     const int w = tid_() >> 6;
     const uint64_t b = __ballot(v);
     if (__lane_id() == 0) sc.wave[2 * WAVES + 0 + 0] = 0u;
-    __syncthreads();
+    lds_barrier();
     if (__lane_id() == 0 && b) atomicOr(&sc.wave[2 * WAVES], 1u);
     (void)w;
-    __syncthreads();
+    lds_barrier();
     const bool r = sc.wave[2 * WAVES] != 0u;
-    __syncthreads();
+    lds_barrier();
     return r;
 }
 
@@ -603,9 +605,9 @@ __device__ __forceinline__ void crt_carry(uint32_t (&v)[C], Get0 get0, Get1 get1
         }
     }
     sc.carry[t] = acc;               // < 2^40
-    __syncthreads();
+    lds_barrier();
     uint64_t cin = t ? sc.carry[t - 1] : 0ull;
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int i = 0; i < C; i++) {
         cin += v[i];
@@ -682,9 +684,9 @@ __device__ __forceinline__ bool sub_vec(uint32_t (&d)[C], const uint32_t (&v)[C]
     }
     // borrow out of the top thread
     if (tid_() == THREADS - 1) sc.wave[2 * WAVES + 1] = bb;
-    __syncthreads();
+    lds_barrier();
     const bool r = sc.wave[2 * WAVES + 1] != 0u;
-    __syncthreads();
+    lds_barrier();
     return r;
 }
 

@@ -168,7 +168,7 @@ __device__ __forceinline__ void stage_in(uint32_t *X, const uint32_t (&v)[G<LOGM
 #pragma unroll
                 for (int i = 0; i < C; i++) X[q * MP + px(p0 + i)] = 0u;
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int i = 0; i < C; i++) {
             const int idx = p0 + i - off;
@@ -184,7 +184,7 @@ __device__ __forceinline__ void stage_in(uint32_t *X, const uint32_t (&v)[G<LOGM
             for (int q = 0; q < NPW; q++) X[q * MP + px(p0 + i)] = x;
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // Split products: the staged operand (natural order, prime 0's buffer) is
@@ -223,7 +223,7 @@ __device__ __forceinline__ void split_twist(uint32_t *X, const Twiddles &T, cons
             X[MP + px(n)] = shoup(x[k], a1[k], b1[k], P1);
         }
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // Inverse of the second half, times w^-n (the recombination's weight).
@@ -426,9 +426,9 @@ __device__ void mul_small_vec(uint32_t (&x)[C], uint32_t s, Scratch &sc)
         acc >>= LIMB_BITS;
     }
     sc.carry[t] = acc;                  // < 2^24 + 2
-    __syncthreads();
+    lds_barrier();
     uint64_t cin = t ? sc.carry[t - 1] : 0ull;
-    __syncthreads();
+    lds_barrier();
 #pragma unroll
     for (int i = 0; i < C; i++) {
         cin += x[i];
@@ -454,7 +454,7 @@ __device__ void shl_vec(uint32_t (&x)[C], long e, uint32_t *X, int M)
     const int p0 = tid_() * C;
 #pragma unroll
     for (int i = 0; i < C; i++) X[px(p0 + i)] = x[i];
-    __syncthreads();
+    lds_barrier();
     const long ls = e / 24;
     const int bs = (int)(e % 24);
 #pragma unroll
@@ -464,7 +464,7 @@ __device__ void shl_vec(uint32_t (&x)[C], long e, uint32_t *X, int M)
         const uint32_t lo = (k - 1 >= 0 && k - 1 < M) ? X[px((int)k - 1)] : 0u;
         x[i] = bs ? (((hi << bs) | (lo >> (24 - bs))) & LIMB_MASK) : hi;
     }
-    __syncthreads();
+    lds_barrier();
 }
 
 // x <- 5^e.
