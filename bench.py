@@ -42,8 +42,8 @@ SMALL_LIMIT = 1 << 16                               # primes below go to the LDS
 # (memory-side atomic ORs of the ~1.7e7 hits).
 TD_TRAFFIC_BYTES = 23.3e6 + 535.5e6
 # Same for one batched squaring step of the 50 000-digit Miller-Rabin batch
-# (256 bigints, 2^14 transforms): 82.4 MB read + 7.3 MB written.
-MR_TRAFFIC_BYTES_PER_STEP = 82.4e6 + 7.3e6
+# (256 bigints, 2^14 transforms): 101.8 MB read + 7.3 MB written.
+MR_TRAFFIC_BYTES_PER_STEP = 101.8e6 + 7.3e6
 
 
 def _load(name, path):
