@@ -130,6 +130,13 @@ def workload3(pf, args) -> dict:
     digits, extrapolated from the timed squaring steps (every candidate of
     the workload is composite at base 2, so one round per bigint)."""
     leg = mr_leg(pf, args.mr_bits or 2048)
+    # the plain path for comparison: one candidate per launch, same kernel
+    cfg = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())["3"]
+    one = pf.MrContext(candidates=[tuple(cfg["candidates"][0]) + (0,)])
+    one.time_squarings(2, 64)
+    one_step_s = one.time_squarings(2, 1024) / 1e3 / 1024
+    one.close()
+    plain_tph = 3600.0 / (one_step_s * 166096)
     return {
         "metric": "Miller-Rabin tests/hour (batched, 50 000 digits)",
         "value": leg["tests_per_hour_extrapolated"], "unit": "tests/hour", "n_gpus": 1,
@@ -139,6 +146,9 @@ def workload3(pf, args) -> dict:
         "config": {"workload": leg["workload"], "transform_len": leg["transform_len"],
                    "sample": leg["sample"], "parallelism": "single GPU"},
         "roofline": leg["roofline"],
+        "plain_one_candidate": {"us_per_squaring": one_step_s * 1e6,
+                                "tests_per_hour": plain_tph,
+                                "batched_over_plain": leg["tests_per_hour_extrapolated"] / plain_tph},
         "device": pf.device_info(),
     }
 

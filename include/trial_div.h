@@ -95,6 +95,16 @@ int trial_div_launch_kj_ev(const uint32_t *d_primes, uint32_t n_primes,
  * 2 = more than 65536 list entries below 2^16 (a duplicated list). */
 int trial_div_check_error(void);
 
+/* Survivor extraction of worker.c:45-54 on the device: the pairs (k, j) of
+ * a K x J bitmap (device pointer, layout as trial_div_gpu_kj) whose bit is
+ * clear, in lexicographic order, written to d_out (device, cap entries) as
+ * (k << 16) | j.  *count (host) receives the number of survivors, which may
+ * exceed cap (then only cap are written).  Synchronous on `stream`.
+ * Returns 0, or -1 (message on stderr).  No counterpart in the reference,
+ * which walks the bitmap on the host. */
+int trial_div_survivors(const uint32_t *d_bits, uint32_t K, uint32_t J,
+                        uint32_t *d_out, uint32_t cap, uint32_t *count, void *stream);
+
 #ifdef __cplusplus
 } /* extern "C" */
 

@@ -49,6 +49,7 @@ def lib() -> C.CDLL:
         "trial_div_launch_kj": (C.c_int, [vp, u32, ll, ll, ll, u32, u32, vp, vp]),
         "trial_div_launch_kj_ev": (C.c_int, [vp, u32, ll, ll, ll, u32, u32, vp, vp, vp, vp]),
         "trial_div_check_error": (C.c_int, []),
+        "trial_div_survivors": (C.c_int, [vp, u32, u32, vp, u32, C.POINTER(C.c_uint32), vp]),
         "cuda_get_device_count": (C.c_int, []),
         "cuda_set_device": (C.c_int, [C.c_int]),
         "prime_list_load": (C.c_int, [C.POINTER(PrimeList), C.c_char_p]),
@@ -134,6 +135,21 @@ def trial_div_gpu_kj(primes: np.ndarray, a: int, b: int, c: int, K: int, J: int,
     if timing:
         return rc, out, (t.upload, t.kernel, t.download)
     return rc, out
+
+
+def survivors_gpu(d_bits, K: int, J: int, cap: int | None = None) -> np.ndarray:
+    """Survivor pairs of a device bitmap (trial_div_survivors): an (n, 2)
+    array of (k, j), lexicographic.  d_bits: a DeviceBuffer or pointer."""
+    ptr = d_bits.ptr if hasattr(d_bits, "ptr") else d_bits
+    cap = K * J if cap is None else cap
+    out = DeviceBuffer(4 * max(cap, 1))
+    n = C.c_uint32()
+    if lib().trial_div_survivors(ptr, K, J, out.ptr, cap, C.byref(n), None) != 0:
+        raise RuntimeError("trial_div_survivors failed")
+    got = min(n.value, cap)
+    packed = out.download(count=got) if got else np.zeros(0, np.uint32)
+    out.close()
+    return np.stack([packed >> 16, packed & 0xFFFF], axis=1).astype(np.int64)
 
 
 class DeviceBuffer:

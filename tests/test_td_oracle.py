@@ -130,3 +130,17 @@ def test_survivor_order_and_k100_gate():
     assert surv == sorted(surv)
     assert len(surv) == 10000 - popcount(bits)
     assert (bits[312] >> 16) == 0
+
+
+def test_pipeline_fixture_survivors_match_oracle():
+    """tests/golden/pipeline_config1.json's survivors are the oracle's for
+    workload 1 (the fixture the GPU pipeline test compares against)."""
+    import hashlib
+    import json
+    from _support import ROOT, numpy_primes, oracle_sieve, survivors
+    fx = json.loads((ROOT / "tests" / "golden" / "pipeline_config1.json").read_text())
+    bits = oracle_sieve(numpy_primes(fx["prime_limit"]), fx["a"], fx["b"], fx["c"], fx["K"], fx["J"])
+    surv = survivors(bits, fx["K"], fx["J"])
+    assert len(surv) == fx["survivor_count"]
+    packed = np.array([(k << 16) | j for k, j in surv], dtype="<u4")
+    assert hashlib.sha256(packed.tobytes()).hexdigest() == fx["survivors_sha256"]
