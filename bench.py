@@ -42,8 +42,8 @@ SMALL_LIMIT = 1 << 16                               # primes below go to the LDS
 # (memory-side atomic ORs of the ~1.7e7 hits).
 TD_TRAFFIC_BYTES = 23.3e6 + 535.5e6
 # Same for one batched squaring step of the 50 000-digit Miller-Rabin batch
-# (256 bigints, 2^14 transforms): 101.8 MB read + 7.3 MB written.
-MR_TRAFFIC_BYTES_PER_STEP = 101.8e6 + 7.3e6
+# (256 bigints, 2^14 transforms): 91.7 MB read + 14.4 MB written.
+MR_TRAFFIC_BYTES_PER_STEP = 91.7e6 + 14.4e6
 
 
 def _load(name, path):
@@ -81,12 +81,16 @@ def cpu_baseline(primes: np.ndarray, triple, budget_s: float) -> dict:
 def mr_ops_per_squaring(logm: int) -> int:
     """INT32 VALU ops of one Montgomery squaring step on one candidate, by
     the transform-count model T(M) = (M/2) log2(M) 12 per transform and
-    prime, ops = 2 (6 T + 24 M) + 60 M (two primes, six transforms each,
-    pointwise and CRT/carry per coefficient), at this build's transform
-    length M = 2^logm (24-bit limbs: 2^14 at 50 000 digits)."""
+    prime.  Per prime: r^2 and T_lo N' are products of M points (a forward
+    and an inverse transform each, the other operand precomputed), m N is a
+    wrapped product of M/2 points (its low half is known), so
+    ops = 2 (4 T(M) + 2 T(M/2) + 20 M) + 50 M, the 20 M and 50 M being the
+    pointwise products and the CRT/carry per coefficient, at this build's
+    transform length M = 2^logm (24-bit limbs: 2^14 at 50 000 digits)."""
     M = 1 << logm
     T = (M // 2) * logm * 12
-    return 2 * (6 * T + 24 * M) + 60 * M
+    Th = (M // 4) * (logm - 1) * 12
+    return 2 * (4 * T + 2 * Th + 20 * M) + 50 * M
 
 
 def mr_leg(pf, n_bits: int) -> dict:
@@ -117,8 +121,8 @@ def mr_leg(pf, n_bits: int) -> dict:
                         "unit": "Tops/s", "frac": achieved / INT32_VALU_PEAK_TOPS,
                         "traffic": MR_TRAFFIC_BYTES_PER_STEP if ctx.log_len == 14 else None,
                         "traffic_note": "HBM bytes per batched squaring step (rocprofv3 "
-                                        "FETCH_SIZE + WRITE_SIZE); algorithmic 67.1e6 "
-                                        "(NTT(N), NTT(N') per bigint)",
+                                        "FETCH_SIZE + WRITE_SIZE); algorithmic 50.3e6 "
+                                        "(NTT(N') and NTT(N) at M/2 points per bigint)",
                         "kernel": f"exp_kernel<{ctx.log_len}>",
                         "ops_per_squaring_step": ops}}
     ctx.close()
@@ -231,7 +235,7 @@ def capacity_estimate(pf, rows, bases_n=1) -> dict:
     busy = 0.0
     for r in rows:
         bits = int(r[0] * 3.3219280948873623)
-        L = (bits + 2 + 23) // 24
+        L = (bits + 2) // 24 + 1
         logm = 14 if 2 * L + 2 <= 1 << 14 else 15
         busy += bits * per_step[logm]
     mean_s = busy / len(rows)               # one slot busy per request

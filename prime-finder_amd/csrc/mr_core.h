@@ -135,6 +135,16 @@ struct Twiddles {
     // (i < 2^14, with quotients), big_half = 2^14, for the half twist.
     const uint32_t *big;
     int big_half;
+    // Tables of the transform of M/2 points (base, pass, half) for the
+    // wrapped products of unsplit lengths.
+    const uint32_t *hbase, *hpass;
+    int hhalf;
+    __device__ __forceinline__ Twiddles halved() const
+    {
+        Twiddles h = *this;
+        h.base = hbase; h.pass = hpass; h.half = hhalf;
+        return h;
+    }
     __device__ __forceinline__ const uint32_t *tw(int q) const { return base + (size_t)q * 4 * half; }
     __device__ __forceinline__ const uint32_t *twq(int q) const { return tw(q) + half; }
     __device__ __forceinline__ const uint32_t *itw(int q) const { return tw(q) + 2 * half; }
@@ -673,6 +683,7 @@ __device__ __forceinline__ bool sub_vec(uint32_t (&d)[C], const uint32_t (&v)[C]
     bool all0 = true;
 #pragma unroll
     for (int i = 0; i < C; i++) all0 &= (d[i] == 0u);
+    const uint32_t own = bb;                        // borrow this thread generates
     // a borrow into this thread passes through iff every limb is 0
     const uint32_t b = block_carry_in(bb != 0u, all0 && bb == 0u, sc);
     bb = b;
@@ -682,8 +693,9 @@ __device__ __forceinline__ bool sub_vec(uint32_t (&d)[C], const uint32_t (&v)[C]
         d[i] = s & LIMB_MASK;
         bb = (s >> 31) & 1u;
     }
-    // borrow out of the top thread
-    if (tid_() == THREADS - 1) sc.wave[2 * WAVES + 1] = bb;
+    // borrow out of the top thread: generated there, or passed through it
+    // (never both: a thread that generates one has a nonzero limb left)
+    if (tid_() == THREADS - 1) sc.wave[2 * WAVES + 1] = own | bb;
     lds_barrier();
     const bool r = sc.wave[2 * WAVES + 1] != 0u;
     lds_barrier();

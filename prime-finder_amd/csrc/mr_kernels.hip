@@ -47,6 +47,7 @@ struct Dev {                    // per-batch device state (kernel argument)
                                 //   lane-interleaved: element i of thread t at i * THREADS + t
     uint32_t *Nf;               // [n][2M] NTT(N) * 2^32 per prime
     uint32_t *Npf;              // [n][2M] NTT(N') * 2^32 per prime
+    uint32_t *Nf2;              // [n][M] NTT of M/2 points of N * 2^32 per prime (HALF lengths)
     uint32_t *one;              // [n][stride] R mod N
     uint32_t *nm1;              // [n][stride] (N - 1) R mod N
     uint32_t *res;              // [n][stride] current residue (Montgomery form)
@@ -67,6 +68,7 @@ __device__ __forceinline__ int slot_id(const Dev &D)
 
 struct Consts {
     CrtConst sq, mul;           // CRT scales after Montgomery square / exact product
+    CrtConst mulh;              // exact product through a transform of M/2 points
     uint32_t r2[2];             // 2^64 mod p (to Montgomery form)
     uint32_t np[2];             // -p^-1 mod 2^32
 };
@@ -88,6 +90,11 @@ struct G {
     static constexpr int TL = SPLIT ? 14 : LOGM;      // log2 of each transform
     static constexpr int H = 1 << TL;
     static constexpr int YSV = SPLIT ? M + H : 0;      // per-slot words of Dev::ysv
+    // Products known modulo B^(M/2) - 1 only (m N in the Montgomery step)
+    // run through transforms of M/2 points: at 2^15 the first split half,
+    // below that a transform of LOGM - 1 (none at the smallest length).
+    static constexpr bool WRAP = LOGM >= 12;
+    static constexpr bool HALF = WRAP && !SPLIT;       // own M/2 transform
 };
 
 template <int LOGM>
@@ -154,12 +161,12 @@ __device__ __forceinline__ void zero_below(uint32_t (&v)[C], int lim)
 
 // Put positions [off, off+len) of v at X_q[0..len), zero elsewhere, for the
 // NPW buffers starting at X.
-// Only positions below H = 2^TL exist in the buffers (the operand has at
-// most L < H limbs).
-template <int LOGM, int NPW>
+// Only positions below H = 2^TLS exist in the buffers of 2^TLS points (the
+// operand has at most L < H limbs).
+template <int LOGM, int NPW, int TLS = G<LOGM>::TL>
 __device__ __forceinline__ void stage_in(uint32_t *X, const uint32_t (&v)[G<LOGM>::C], int off, int len)
 {
-    constexpr int C = G<LOGM>::C, MP = Lds<G<LOGM>::TL>::MP, H = G<LOGM>::H;
+    constexpr int C = G<LOGM>::C, MP = Lds<TLS>::MP, H = 1 << TLS;
     const int p0 = tid_() * C;
     if (off != 0) {
         if (p0 < H) {
@@ -256,27 +263,62 @@ __device__ __forceinline__ void split_unweight(uint32_t *X, const Twiddles &T)
 // precomputed transform Bf.  a and out may alias.  Split products (M = 2^15)
 // park the first half's inverse u in ysave (M words, lane-interleaved by
 // the reading thread) and the staged operand at ysave + M (H words).
+// wrap (split products only): out = the cyclic product of length H, i.e.
+// A * B mod (x^H - 1) evaluated at x = 2^24, normalised over [0, M) -- the
+// first half alone, whose inverse transform is that cyclic product.
 template <int LOGM>
-__device__ void tproduct(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
+__device__ __forceinline__ void tproduct(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
                          const uint32_t (&a)[G<LOGM>::C], int off, int len,
                          const uint32_t *Bf, uint32_t (&out)[G<LOGM>::C], int keep,
-                         uint32_t *ysave)
+                         uint32_t *ysave, bool wrap = false)
 {
     constexpr int C = G<LOGM>::C, M = G<LOGM>::M, TL = G<LOGM>::TL, MP = Lds<TL>::MP;
     const CrtConst k = Bf ? K.mul : K.sq;
     const int p0 = tid_() * C;
     const uint32_t *X0 = sm.X, *X1 = sm.X + MP;
-    stage_in<LOGM, 2>(sm.X, a, off, len);
     if constexpr (!G<LOGM>::SPLIT) {
-        convolve<LOGM, 2>(sm.X, T, 0, Bf, K.np);
-        crt_carry<C>(out, [&](int i) { return X0[px(p0 + i)]; }, [&](int i) { return X1[px(p0 + i)]; },
-                     k, keep, sm.sc);
+        // unsplit: M points, or M/2 points (buffers at the smaller pitch) for
+        // a wrapped product
+        bool half = false;
+        if constexpr (G<LOGM>::HALF) {
+            if (wrap) {
+                half = true;
+                stage_in<LOGM, 2, LOGM - 1>(sm.X, a, off, len);
+                convolve<LOGM - 1, 2>(sm.X, T.halved(), 0, Bf, K.np);
+            }
+        }
+        if (!half) {
+            stage_in<LOGM, 2>(sm.X, a, off, len);
+            convolve<LOGM, 2>(sm.X, T, 0, Bf, K.np);
+        }
+        // a thread's run lies wholly below or above M/2; loads stay in the
+        // buffer either way (unconditional)
+        const bool in = !half || p0 < M / 2;
+        const int b0 = in ? p0 : 0;
+        const uint32_t *Y1 = half ? sm.X + Lds<G<LOGM>::HALF ? LOGM - 1 : LOGM>::MP : X1;
+        crt_carry<C>(out,
+                     [&](int i) {
+                         const uint32_t y = X0[px(b0 + i)];
+                         return in ? y : 0u;
+                     },
+                     [&](int i) {
+                         const uint32_t y = Y1[px(b0 + i)];
+                         return in ? y : 0u;
+                     },
+                     half ? K.mulh : k, keep, sm.sc);
     } else {
         constexpr int H = G<LOGM>::H, HALF_T = THREADS / 2;   // threads covering [0, H)
         static_assert(C * HALF_T == H, "split layout");
+        stage_in<LOGM, 2>(sm.X, a, off, len);
         uint32_t *asave = ysave + M;
-        split_save<LOGM>(sm.X, asave);
+        if (!wrap) split_save<LOGM>(sm.X, asave);
         convolve<TL, 2>(sm.X, T, 0, Bf, K.np, M);                      // even frequencies
+        if (wrap) {
+            const bool lo = p0 < H;
+            crt_carry<C>(out, [&](int i) { return lo ? X0[px(p0 + i)] : 0u; },
+                         [&](int i) { return lo ? X1[px(p0 + i)] : 0u; }, K.mulh, keep, sm.sc);
+            return;
+        }
         {   // u -> ysave: element i of position block t' (n = t' C + i) at
             // q H + i HALF_T + t'; thread t writes half of t mod HALF_T's run
             const int t = tid_(), tp = t & (HALF_T - 1), i0 = (t / HALF_T) * (C / 2);
@@ -309,7 +351,7 @@ __device__ void tproduct(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
 // (split products: the two halves' transforms side by side).  save: H
 // scratch words for split products.
 template <int LOGM>
-__device__ void to_fft(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
+__device__ __forceinline__ void to_fft(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
                        const uint32_t (&v)[G<LOGM>::C], int len, uint32_t *dst, uint32_t *save)
 {
     constexpr int M = G<LOGM>::M, TL = G<LOGM>::TL, H = G<LOGM>::H, MP = Lds<TL>::MP;
@@ -326,6 +368,24 @@ __device__ void to_fft(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
         }
         __syncthreads();
     }
+}
+
+// dst[0..M) = transform of M/2 points of positions [0, len) of v (len <
+// M/2), times 2^32, per prime (prime q's at dst + q M/2).
+template <int LOGM>
+__device__ __forceinline__ void to_fft_half(Smem<LOGM> &sm, const Twiddles &T, const Consts &K,
+                            const uint32_t (&v)[G<LOGM>::C], int len, uint32_t *dst)
+{
+    constexpr int WL = LOGM - 1, HW = 1 << WL, WP = Lds<WL>::MP;
+    stage_in<LOGM, 2, WL>(sm.X, v, 0, len);
+    forward<WL, 2>(sm.X, T.halved(), 0, K.np);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const uint32_t p = prime(q), np = K.np[q], r2 = K.r2[q];
+        for (int i = tid_(); i < HW; i += THREADS)
+            dst[(size_t)q * HW + i] = mont(sm.X[q * WP + px(i)], r2, p, np);
+    }
+    __syncthreads();
 }
 
 // r <- r - N when r >= N; N read from Nl at layout offset off, or (Nil
@@ -348,7 +408,7 @@ __device__ __forceinline__ void reduce_once(uint32_t (&r)[C], const uint32_t *Nl
 // additions).  The multiplicand waits in LDS (stash, padded, MP words) so
 // that at most three limb vectors are live.
 template <int C>
-__device__ void mulsmall_modN(uint32_t (&r)[C], uint32_t s, const uint32_t *Nl, int L, int off,
+__device__ __forceinline__ void mulsmall_modN(uint32_t (&r)[C], uint32_t s, const uint32_t *Nl, int L, int off,
                               uint32_t *stash, Scratch &sc, const uint32_t *Nil = nullptr)
 {
     const int p0 = tid_() * C;
@@ -368,12 +428,64 @@ __device__ void mulsmall_modN(uint32_t (&r)[C], uint32_t s, const uint32_t *Nl, 
     }
 }
 
+// Montgomery tail after a wrapped m N (W = m N mod (B^H - 1), B = 2^24,
+// H = M/2 > L, normalised over [0, M) in r; T parked in park over [0, 2L)):
+// T + m N = t B^L with t < 2N, so V = T + W == t B^L (mod B^H - 1).  The
+// residue of t B^L has t's limbs below position H - L at [L, H) and the
+// rest wrapped to [0, 2L + 1 - H), which ends at or below L.  So: fold V
+// (limbs [H, M) added at 0, one end-around carry), then unwrap to t at
+// [L, 2L + 1) -- the high layout.  V == 0 (mod B^H - 1) only when t = 0,
+// and then V = 0 exactly (T = 0 forces m = 0), so the fold never yields the
+// all-ones form of zero.
+template <int LOGM>
+__device__ __forceinline__ void wrap_tail(Smem<LOGM> &sm, uint32_t (&r)[G<LOGM>::C], const uint32_t *park, int L)
+{
+    constexpr int C = G<LOGM>::C, M = G<LOGM>::M, H = M / 2;
+    static_assert(sizeof(sm.X) / 4 >= M + M / 64, "LDS staging of M limbs");
+    uint32_t *Y = sm.X;                  // the transform buffers hold M padded words
+    const int p0 = tid_() * C;
+    uint32_t t[C];
+    load_own<C>(t, park, 0, 2 * L);
+    add_vec<C>(r, t, 0u, sm.sc);         // V = T + W, within [0, M)
+#pragma unroll
+    for (int i = 0; i < C; i++) Y[px(p0 + i)] = r[i];
+    lds_barrier();
+#pragma unroll
+    for (int i = 0; i < C; i++) {          // unconditional loads (indexes wrap mod M)
+        const bool lo = p0 + i < H;
+        const uint32_t y = Y[px((p0 + i + H) & (M - 1))];
+        t[i] = lo ? y : 0u;
+        if (!lo) r[i] = 0u;
+    }
+    lds_barrier();
+    add_vec<C>(r, t, 0u, sm.sc);         // V_lo + V_hi < 2 B^H: a carry lands at H
+    bool top = false;
+#pragma unroll
+    for (int i = 0; i < C; i++)
+        if (p0 + i == H && r[i] != 0u) { top = true; r[i] = 0u; }
+    if (block_any(top, sm.sc)) {         // B^H == 1: the carry re-enters at 0
+#pragma unroll
+        for (int i = 0; i < C; i++) t[i] = 0u;
+        add_vec<C>(r, t, 1u, sm.sc);     // stays below B^H - 1
+    }
+#pragma unroll
+    for (int i = 0; i < C; i++) Y[px(p0 + i)] = r[i];
+    lds_barrier();
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+        const int pos = p0 + i;
+        const uint32_t y = Y[px((pos - H) & (M - 1))];
+        r[i] = pos < L ? 0u : pos < H ? r[i] : pos <= 2 * L ? y : 0u;
+    }
+    lds_barrier();
+}
+
 // Montgomery: r <- A * B * R^-1 mod N, output in high layout (positions
 // [L, 2L]).  A is positions [off, off+L) of r; B as in tproduct.  The high
 // half of T = A*B waits in the candidate's scratch area while m and m*N are
 // formed, so only two limb vectors are live across a transform.
 template <int LOGM>
-__device__ void montmul(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, const Dev &D, int c,
+__device__ __forceinline__ void montmul(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, const Dev &D, int c,
                         uint32_t (&r)[G<LOGM>::C], int off, const uint32_t *Bf)
 {
     constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
@@ -384,38 +496,54 @@ __device__ void montmul(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, cons
     const uint32_t *Nl = D.N + (size_t)c * D.stride;
     const uint32_t *Npf = D.Npf + (size_t)c * 2 * M;
     const uint32_t *Nf = D.Nf + (size_t)c * 2 * M;
-    bool cbit = false;
-    // stage 0: r = T = A*B (< R^2); stage 1: r = m = T_lo N' mod R;
-    // stage 2: r = m N.  One product instance keeps the code (and its
-    // register allocation) single.
+    if constexpr (G<LOGM>::WRAP) {
+        // stage 0: r = T = A*B (< R^2); stage 1: r = m = T_lo N' mod R;
+        // stage 2: r = W = m N mod (x^(M/2) - 1) at x = 2^24 (a wrapped
+        // product: half the transform work of the full m N, whose low half
+        // is known).
+        const uint32_t *Nw = G<LOGM>::SPLIT ? Nf : D.Nf2 + (size_t)c * M;
 #pragma unroll 1
-    for (int stage = 0; stage < 3; stage++) {
-        const uint32_t *B = stage == 0 ? Bf : stage == 1 ? Npf : Nf;
-        tproduct<LOGM>(sm, T, K, r, stage == 0 ? off : 0, L, B, r, stage == 1 ? L : M, ysave);
-        if (stage == 0) {
-            bool nz = false;
-#pragma unroll
-            for (int i = 0; i < C; i++) nz |= (p0 + i < L) && r[i] != 0u;
-            cbit = block_any(nz, sm.sc);
-            save_own<C>(r, park, L, 2 * L + 1);                // T_hi
+        for (int stage = 0; stage < 3; stage++) {
+            const uint32_t *B = stage == 0 ? Bf : stage == 1 ? Npf : Nw;
+            tproduct<LOGM>(sm, T, K, r, stage == 0 ? off : 0, L, B, r, stage == 1 ? L : M, ysave,
+                           stage == 2);
+            if (stage == 0) save_own<C>(r, park, 0, 2 * L);    // T
         }
-    }
-    zero_below<C>(r, L);
-    {
-        uint32_t t[C];
-        load_own<C>(t, park, L, 2 * L + 1);
-        // + [T_lo != 0] at position L: T_hi's lowest limb is below 2^24, so
-        // adding 1 cannot overflow the 32-bit lane before add_vec normalises
+        wrap_tail<LOGM>(sm, r, park, L);
+    } else {
+        bool cbit = false;
+        // stage 0: r = T = A*B (< R^2); stage 1: r = m = T_lo N' mod R;
+        // stage 2: r = m N.  One product instance keeps the code (and its
+        // register allocation) single.
+#pragma unroll 1
+        for (int stage = 0; stage < 3; stage++) {
+            const uint32_t *B = stage == 0 ? Bf : stage == 1 ? Npf : Nf;
+            tproduct<LOGM>(sm, T, K, r, stage == 0 ? off : 0, L, B, r, stage == 1 ? L : M, ysave);
+            if (stage == 0) {
+                bool nz = false;
 #pragma unroll
-        for (int i = 0; i < C; i++) t[i] += (cbit && p0 + i == L) ? 1u : 0u;
-        add_vec<C>(r, t, 0u, sm.sc);                           // T_hi + (mN)_hi + c
+                for (int i = 0; i < C; i++) nz |= (p0 + i < L) && r[i] != 0u;
+                cbit = block_any(nz, sm.sc);
+                save_own<C>(r, park, L, 2 * L + 1);            // T_hi
+            }
+        }
+        zero_below<C>(r, L);
+        {
+            uint32_t t[C];
+            load_own<C>(t, park, L, 2 * L + 1);
+            // + [T_lo != 0] at position L: T_hi's lowest limb is below 2^24, so
+            // adding 1 cannot overflow the 32-bit lane before add_vec normalises
+#pragma unroll
+            for (int i = 0; i < C; i++) t[i] += (cbit && p0 + i == L) ? 1u : 0u;
+            add_vec<C>(r, t, 0u, sm.sc);                       // T_hi + (mN)_hi + c
+        }
     }
     reduce_once<C>(r, Nl, L, L, sm.sc, D.Nh + (size_t)c * M);
 }
 
 // x <- x * s (plain integer product), 1 <= s < 2^24.
 template <int C>
-__device__ void mul_small_vec(uint32_t (&x)[C], uint32_t s, Scratch &sc)
+__device__ __forceinline__ void mul_small_vec(uint32_t (&x)[C], uint32_t s, Scratch &sc)
 {
     const int t = tid_();
     uint64_t acc = 0;
@@ -449,7 +577,7 @@ __device__ void mul_small_vec(uint32_t (&x)[C], uint32_t s, Scratch &sc)
 
 // x <- x * 2^e through the padded LDS buffer X (MP words).
 template <int C>
-__device__ void shl_vec(uint32_t (&x)[C], long e, uint32_t *X, int M)
+__device__ __forceinline__ void shl_vec(uint32_t (&x)[C], long e, uint32_t *X, int M)
 {
     const int p0 = tid_() * C;
 #pragma unroll
@@ -469,7 +597,7 @@ __device__ void shl_vec(uint32_t (&x)[C], long e, uint32_t *X, int M)
 
 // x <- 5^e.
 template <int LOGM>
-__device__ void pow5(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, uint32_t (&x)[G<LOGM>::C],
+__device__ __forceinline__ void pow5(Smem<LOGM> &sm, const Twiddles &T, const Consts &K, uint32_t (&x)[G<LOGM>::C],
                      long e, uint32_t *ysave)
 {
     constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
@@ -637,6 +765,7 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
     uint32_t *ysv = D.ysv + (size_t)c * G<LOGM>::YSV;
     uint32_t *fsave = ysv + M;                       // split transforms' scratch
     to_fft<LOGM>(sm, T, K, nv, L, Nf, fsave);
+    if constexpr (G<LOGM>::HALF) to_fft_half<LOGM>(sm, T, K, nv, L, D.Nf2 + (size_t)c * M);
 
     // N' = -N^-1 mod R by Newton: y <- y (2 + N y), doubling precision.
     {
@@ -878,6 +1007,7 @@ Consts make_consts(int logm)
         k.c2 = P0_INV_MOD_P1; k.c2q = h_shoupq(k.c2, P1);
     };
     fill(K.mul, minv0, minv1);
+    fill(K.mulh, (uint32_t)h_pow(M / 2, P0 - 2, P0), (uint32_t)h_pow(M / 2, P1 - 2, P1));
     fill(K.sq, (uint32_t)((uint64_t)minv0 * r32_0 % P0), (uint32_t)((uint64_t)minv1 * r32_1 % P1));
     K.r2[0] = (uint32_t)((uint64_t)r32_0 * r32_0 % P0);
     K.r2[1] = (uint32_t)((uint64_t)r32_1 * r32_1 % P1);
@@ -889,6 +1019,11 @@ Consts make_consts(int logm)
 constexpr int MIN_LOGM = 11, MAX_LOGM = 15;
 int g_min_logm = MIN_LOGM;      // floor for new contexts (mr_set_min_log_len)
 constexpr bool split_logm(int logm) { return logm > 14; }
+constexpr bool half_logm(int logm) { return logm >= 12 && !split_logm(logm); }
+static_assert(split_logm(14) == G<14>::SPLIT && split_logm(15) == G<15>::SPLIT &&
+              half_logm(11) == G<11>::HALF && half_logm(12) == G<12>::HALF &&
+              half_logm(14) == G<14>::HALF && half_logm(15) == G<15>::HALF,
+              "host and device agree on the product geometry");
 
 struct TwCache {
     uint32_t *tw[MAX_LOGM + 1] = {};
@@ -1007,10 +1142,11 @@ int ctx_alloc(mr_ctx *x, void **p, size_t bytes)
 
 constexpr int TMAX = 24;       // sparse decimal terms per candidate
 
-// Limbs and log2 transform length for numbers of at most `bits` bits.
+// Limbs and log2 transform length for numbers of at most `bits` bits:
+// 24 L > bits, so that 2N < R = 2^(24 L).
 void geometry(long bits, int min_logm, int *L_out, int *logm_out)
 {
-    int L = (int)((bits + 23) / 24);
+    int L = (int)(bits / 24 + 1);
     if (L < 4) L = 4;
     int logm = min_logm;
     while ((1 << logm) < 2 * L + 2) logm++;
@@ -1047,6 +1183,7 @@ int ctx_alloc_all(mr_ctx *x, long bits, int min_logm)
     MR_ALLOC(Nh, uint32_t, n * M);
     MR_ALLOC(Nf, uint32_t, n * 2 * M);
     MR_ALLOC(Npf, uint32_t, n * 2 * M);
+    MR_ALLOC(Nf2, uint32_t, half_logm(logm) ? n * M : 1);
     MR_ALLOC(one, uint32_t, n * x->stride);
     MR_ALLOC(nm1, uint32_t, n * x->stride);
     MR_ALLOC(res, uint32_t, n * x->stride);
@@ -1077,6 +1214,16 @@ int ctx_alloc_all(mr_ctx *x, long bits, int min_logm)
     x->T.big = btw;
     x->T.big_half = M / 2;
     x->T.half = (1 << tl) / 2;
+    x->T.hbase = tw;
+    x->T.hpass = ptw;
+    x->T.hhalf = x->T.half;
+    if (half_logm(logm)) {      // wrapped products of M/2 points
+        if (get_twiddles(x->device, logm - 1, x->st, &tw, &ptw) != 0)
+            return hip_fail(hipErrorUnknown, "twiddles");
+        x->T.hbase = tw;
+        x->T.hpass = ptw;
+        x->T.hhalf = M / 4;
+    }
     x->K = make_consts(logm);
     return 0;
 }
@@ -1512,7 +1659,7 @@ struct Req {
 int64_t slot_bytes(int logm)
 {
     const int64_t M = 1ll << logm;
-    return 4 * (4 * M + 7 * M + (split_logm(logm) ? M + M / 2 : 0)) + 32;
+    return 4 * (4 * M + 7 * M + (split_logm(logm) ? M + M / 2 : 0) + (half_logm(logm) ? M : 0)) + 32;
 }
 
 struct Pool {
@@ -1586,9 +1733,10 @@ int pool_new(mr_sched *S, int logm, Pool **out)
     p->n = n;
     p->x = ctx_new(1, n, S->device);
     int rc = p->x ? 0 : -2;
-    // numbers of up to (2^logm - 2) / 2 limbs: the pool's transform length
+    // numbers of up to (2^logm - 2) / 2 limbs (24 lmax - 1 bits): the pool's
+    // transform length
     const long lmax = std::min<long>(((1l << logm) - 2) / 2, MAX_LIMBS);
-    if (rc == 0) rc = ctx_alloc_all(p->x, 24l * lmax, logm);
+    if (rc == 0) rc = ctx_alloc_all(p->x, 24l * lmax - 1, logm);
     if (rc == 0 && p->x->logm != logm) rc = -1;
     auto dev = [&](void **q, size_t b) { if (rc == 0 && ctx_alloc(p->x, q, b) != 0) rc = -2; };
     dev((void **)&p->d_list_run, 4 * (size_t)n);

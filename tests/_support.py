@@ -209,6 +209,18 @@ def gmp_version() -> str:
     return _mr_oracle().mr_oracle_gmp_version().decode()
 
 
+def inv_pow2_mod(n: int, R: int) -> int:
+    """R^-1 mod n for odd n and R a power of two: u = n^-1 mod R by Newton
+    (y <- y (2 - n y), doubling the bits), n u = 1 + k R, so R (-k) == 1
+    (mod n).  Far faster than pow(R, -1, n) at 10^5 bits."""
+    kb, y = 1, 1
+    while (1 << kb) < R:
+        kb *= 2
+        y = y * (2 - n * y) % (1 << kb)
+    k = (n * (y % R) - 1) // R
+    return (-k) % n
+
+
 def sprp_python(N: int, base: int):
     """Textbook strong-probable-prime round with Python integers."""
     d, s = N - 1, 0

@@ -7,7 +7,7 @@ import sys
 
 import pytest
 
-from _support import gmp_sprp, sprp_python
+from _support import gmp_sprp, inv_pow2_mod, sprp_python
 
 pytestmark = pytest.mark.gpu
 
@@ -39,18 +39,24 @@ def test_product_exact(gpu, bits):
     ctx.close()
 
 
-@pytest.mark.parametrize("bits", [64, 1500, 30000, 166097])
+@pytest.mark.parametrize("bits", [64, 96, 1500, 30000, 166097, 332193, 384000])
 def test_montmul_and_mulsmall(gpu, bits):
+    """Montgomery products (a zero operand, N - 1 squared) and small-factor
+    products against Python, up to the largest supported modulus (2^15
+    transforms, where m N is a wrapped product).  96 and 384 000 bits fill
+    whole 32-bit words and whole 24-bit limbs at once."""
     rng = random.Random(bits + 1)
     mods = [rand_odd(rng, bits) for _ in range(4)]
     ctx = gpu.MrContext(mods)
+    say("montmul", bits, "bits, L", ctx.limbs, "logM", ctx.log_len)
     R = ctx.R
     a = [rng.randrange(n) for n in mods]
     b = [rng.randrange(n) for n in mods]
     a[1], b[1] = mods[1] - 1, mods[1] - 1
+    a[2] = 0
     got = ctx.debug_montmul(a, b)
     for x, y, n, g in zip(a, b, mods, got):
-        assert g == x * y * pow(R, -1, n) % n
+        assert g == x * y * inv_pow2_mod(n, R) % n
     for s in (2, 3, 7, 53, (1 << 24) - 1):
         got = ctx.debug_mulsmall(a, s)
         for x, n, g in zip(a, mods, got):
@@ -232,7 +238,7 @@ def test_every_transform_length(gpu, logm):
     b = [rng.randrange(n) for n in mods]
     assert ctx.debug_product(a, b) == [x * y for x, y in zip(a, b)]
     R = ctx.R
-    assert ctx.debug_montmul(a, b) == [x * y * pow(R, -1, n) % n for x, y, n in zip(a, b, mods)]
+    assert ctx.debug_montmul(a, b) == [x * y * inv_pow2_mod(n, R) % n for x, y, n in zip(a, b, mods)]
     alive, res = ctx.run([2, 3, 5, 7])
     for i, n in enumerate(mods):
         exp_alive = True
@@ -242,6 +248,37 @@ def test_every_transform_length(gpu, logm):
                 assert res[i][bi] == r
             exp_alive = exp_alive and ok
         assert alive[i] == exp_alive
+    ctx.close()
+
+
+@pytest.mark.parametrize("bits", [49120, 98272, 196576])
+def test_largest_modulus_per_length(gpu, bits):
+    """The largest modulus of each unsplit transform length (L = M/2 - 1
+    limbs: 2^12, 2^13, 2^14; the bit lengths are whole 32-bit words).  There
+    the high layout's top limbs sit in the last thread's run, which once hid
+    the final subtraction's borrow, and the wrapped m N product of M/2
+    points has the least room.  Montgomery and small-factor products
+    against Python (the reductions of the exponentiation loop are the same
+    code); at 2^12 also full base-2 rounds against GMP (GMP takes minutes
+    per round at the larger sizes)."""
+    rng = random.Random(bits)
+    mods = [rand_odd(rng, bits) for _ in range(3)]
+    mods[2] = (1 << bits) - 1 - 2 * rng.getrandbits(8)      # long runs of ones
+    ctx = gpu.MrContext(mods)
+    say("largest modulus", bits, "bits, L", ctx.limbs, "logM", ctx.log_len)
+    assert 2 * ctx.limbs + 2 == 1 << ctx.log_len
+    a = [rng.randrange(n) for n in mods]
+    b = [rng.randrange(n) for n in mods]
+    a[0], b[0] = mods[0] - 1, mods[0] - 1
+    R = ctx.R
+    assert ctx.debug_montmul(a, b) == [x * y * inv_pow2_mod(n, R) % n for x, y, n in zip(a, b, mods)]
+    for s in (3, (1 << 24) - 1):
+        assert ctx.debug_mulsmall(a, s) == [x * s % n for x, n in zip(a, mods)]
+    if ctx.log_len == 12:
+        alive, res = ctx.run([2])
+        for i, n in enumerate(mods[:2]):
+            ok2, r2, _ = gmp_sprp(n, 2)
+            assert res[i][0] == r2 and alive[i] == ok2
     ctx.close()
 
 
@@ -263,7 +300,7 @@ def test_100k_digit_arithmetic(gpu):
     a[0] = mods[0] - 1
     R = ctx.R
     assert ctx.debug_product(a, b) == [x * y for x, y in zip(a, b)]
-    assert ctx.debug_montmul(a, b) == [x * y * pow(R, -1, n) % n for x, y, n in zip(a, b, mods)]
+    assert ctx.debug_montmul(a, b) == [x * y * inv_pow2_mod(n, R) % n for x, y, n in zip(a, b, mods)]
     ctx.close()
 
 
