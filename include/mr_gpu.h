@@ -132,6 +132,26 @@ int mr_sched_results(mr_sched_t *s, mr_result_t *out, int cap);
 int mr_sched_stats(const mr_sched_t *s, mr_sched_stats_t *out);
 void mr_sched_destroy(mr_sched_t *s);
 
+/* Checkpoint and resume of a serving scheduler.  The reference's master
+ * dumps its pending Phase-2 queue as "a b c k j" lines (master.c:279-369,
+ * checkpoint_save_phase2 / checkpoint_load_phase2) and restarts every test
+ * from scratch; here the candidates already admitted keep their progress.
+ *
+ * mr_sched_checkpoint waits for the work in flight (launching nothing new),
+ * then writes every undecided candidate to the text file at path: its tag,
+ * candidate, completed rounds, and, once admitted, the index of the base in
+ * progress, the next exponent bit and the residue (Montgomery form, 24-bit
+ * limbs).  Results already decided stay in mr_sched_results.  The scheduler
+ * can go on pumping afterwards.  Returns the number of candidates written,
+ * or -1 (file error), -2 (HIP error).
+ *
+ * mr_sched_restore queues the candidates of such a file on s, which must
+ * have been created with the same list of bases; admitted ones resume at
+ * their saved exponent bit.  Returns the number restored, or -1 (unreadable
+ * or malformed file, other bases, invalid candidate; nothing queued). */
+int mr_sched_checkpoint(mr_sched_t *s, const char *path);
+int mr_sched_restore(mr_sched_t *s, const char *path);
+
 /* One call over a whole list: verdict[i] (1 / 0) and rounds[i] (nullable)
  * for cands[i].  Returns 0, -1 (invalid candidate) or -2 (HIP error). */
 int mr_test_batch(const mr_candidate_t *cands, int n, const uint32_t *bases, int n_bases,

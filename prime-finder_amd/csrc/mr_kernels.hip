@@ -25,6 +25,7 @@
 #include <string.h>
 #include <time.h>
 #include <algorithm>
+#include <string>
 #include <new>
 #include <type_traits>
 #include <vector>
@@ -645,6 +646,22 @@ __global__ void fill_i32_kernel(int32_t *p, int n, int v)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// Admission of candidates restored from a checkpoint: after build, setup
+// and a first round start, put back the base in progress and the next
+// exponent bit (the residue was copied into D.res beforehand).
+// info[3 i ..]: slot, base index, exponent bit.
+__global__ void resume_kernel(Dev D, const int32_t *info, int n, const uint32_t *bases)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int c = info[3 * i], bi = info[3 * i + 1];
+    D.bidx[c] = bi;
+    D.base[c] = bases[bi];
+    D.bitpos[c] = info[3 * i + 2];
+    D.alive[c] = 1;
+    D.done[c] = 0;
 }
 
 __global__ void init_alive_kernel(Dev D)
@@ -1652,6 +1669,10 @@ struct Req {
     int L, logm;
     int rounds;
     int64_t terms[TMAX * 2];
+    // restored from a checkpoint after admission: the base in progress, the
+    // next exponent bit and the residue (L limbs) to resume from
+    int resume_bidx = -1, resume_bitpos = 0;
+    std::vector<uint32_t> resume_res;
 };
 
 // Bytes of device memory per slot of a pool of transform length 2^logm
@@ -1670,6 +1691,9 @@ struct Pool {
     int32_t *d_list_run = nullptr, *d_list_adm = nullptr, *d_Ls = nullptr;
     int64_t *d_terms = nullptr;
     uint32_t *d_bases = nullptr;
+    int32_t *d_resume = nullptr;    // [n][3] slot, base index, exponent bit
+    int32_t *h_resume = nullptr;
+    uint32_t *h_resres = nullptr;   // [n][stride] residues to resume (pinned, on first use)
     // pinned host mirrors
     int32_t *h_list_run = nullptr, *h_list_adm = nullptr, *h_Ls = nullptr;
     int32_t *h_bitpos = nullptr, *h_sval = nullptr;
@@ -1711,7 +1735,7 @@ void pool_free(Pool *p)
     if (p->ev_adm) (void)hipEventDestroy(p->ev_adm);
     if (p->st_run) (void)hipStreamDestroy(p->st_run);
     void *hp[] = {p->h_list_run, p->h_list_adm, p->h_Ls, p->h_bitpos, p->h_sval,
-                  p->h_terms, p->h_alive, p->h_done, p->h_err};
+                  p->h_terms, p->h_alive, p->h_done, p->h_err, p->h_resume, p->h_resres};
     for (void *q : hp) if (q) (void)hipHostFree(q);
     if (p->x) mr_ctx_destroy(p->x);       // frees the device arrays it owns
     delete p;
@@ -1744,6 +1768,7 @@ int pool_new(mr_sched *S, int logm, Pool **out)
     dev((void **)&p->d_Ls, 4 * (size_t)n);
     dev((void **)&p->d_terms, sizeof(int64_t) * TMAX * 2 * (size_t)n);
     dev((void **)&p->d_bases, 4 * (size_t)S->n_bases);
+    dev((void **)&p->d_resume, 12 * (size_t)n);
     auto host = [&](void **q, size_t b) {
         if (rc == 0 && hipHostMalloc(q, b ? b : 4, hipHostMallocDefault) != hipSuccess) rc = -2;
     };
@@ -1756,6 +1781,7 @@ int pool_new(mr_sched *S, int logm, Pool **out)
     host((void **)&p->h_alive, (size_t)n);
     host((void **)&p->h_done, (size_t)n);
     host((void **)&p->h_err, 4);
+    host((void **)&p->h_resume, 12 * (size_t)n);
     if (rc == 0 && (hipStreamCreateWithFlags(&p->st_run, hipStreamNonBlocking) != hipSuccess ||
                     hipEventCreateWithFlags(&p->ev_run, hipEventDisableTiming) != hipSuccess ||
                     hipEventCreateWithFlags(&p->ev_adm, hipEventDisableTiming) != hipSuccess))
@@ -1832,6 +1858,7 @@ int launch_admin(mr_sched *S, Pool *p)
     for (int c = 0; c < p->n; c++)
         if (p->state[c] == SL_END) p->adm_list.push_back(c);
     const int n_end = (int)p->adm_list.size();
+    int n_res = 0;          // restored candidates among the admitted
     for (int c = 0; c < p->n && p->pend_head < p->pending.size() && S->active < S->max_active; c++) {
         if (p->state[c] != SL_FREE) continue;
         const int r = p->pending[p->pend_head++];
@@ -1839,6 +1866,18 @@ int launch_admin(mr_sched *S, Pool *p)
         const int i = (int)p->adm_list.size() - n_end;
         memcpy(p->h_terms + (size_t)i * TMAX * 2, q.terms, sizeof(q.terms));
         p->h_Ls[i] = q.L;
+        if (q.resume_bidx >= 0) {
+            if (!p->h_resres &&
+                hipHostMalloc(&p->h_resres, 4 * (size_t)p->n * p->x->stride, hipHostMallocDefault) != hipSuccess) {
+                p->h_resres = nullptr;
+                return hip_fail(hipErrorOutOfMemory, "resume staging");
+            }
+            memcpy(p->h_resres + (size_t)n_res * p->x->stride, q.resume_res.data(), 4 * (size_t)q.L);
+            p->h_resume[3 * n_res] = c;
+            p->h_resume[3 * n_res + 1] = q.resume_bidx;
+            p->h_resume[3 * n_res + 2] = q.resume_bitpos;
+            n_res++;
+        }
         q.t_admit = mono_now();
         p->req[c] = r;
         p->adm_list.push_back(c);
@@ -1878,6 +1917,17 @@ int launch_admin(mr_sched *S, Pool *p)
     });
     if (rc) return rc;
     MR_CHECK(hipGetLastError());
+    if (n_res) {        // restored candidates: residue and position after the round start
+        for (int i = 0; i < n_res; i++) {
+            const int c = p->h_resume[3 * i];
+            MR_CHECK(hipMemcpyAsync(p->x->d.res + (size_t)c * p->x->stride,
+                                    p->h_resres + (size_t)i * p->x->stride,
+                                    4 * (size_t)S->reqs[p->req[c]].L, hipMemcpyHostToDevice, st));
+        }
+        MR_CHECK(hipMemcpyAsync(p->d_resume, p->h_resume, 12 * (size_t)n_res, hipMemcpyHostToDevice, st));
+        resume_kernel<<<(n_res + 63) / 64, 64, 0, st>>>(p->x->d, p->d_resume, n_res, bases);
+        MR_CHECK(hipGetLastError());
+    }
     const size_t n = (size_t)p->n;
     MR_CHECK(hipMemcpyAsync(p->h_alive, p->x->d.alive, n, hipMemcpyDeviceToHost, st));
     MR_CHECK(hipMemcpyAsync(p->h_done, p->x->d.done, n, hipMemcpyDeviceToHost, st));
@@ -1944,40 +1994,43 @@ extern "C" mr_sched_t *mr_sched_create(const uint32_t *bases, int n_bases, int d
     return S;
 }
 
-extern "C" int mr_sched_submit(mr_sched_t *S, const mr_candidate_t *cands, const int64_t *tags, int n)
+namespace {
+
+// A request for candidate cd: sparse terms and geometry.  False (with a
+// message) when the candidate cannot be formed or is too large.
+bool make_req(const mr_candidate_t &cd, int64_t tag, Req &q)
 {
-    if (!S || (!cands && n) || n < 0) return -1;
-    std::vector<Req> add((size_t)n);
     std::vector<std::pair<int64_t, int64_t>> t;
-    for (int i = 0; i < n; i++) {
-        Req &q = add[i];
-        int64_t dg = 0;
-        if (!candidate_terms(cands[i], t, &dg) || (int)t.size() > TMAX) {
-            fprintf(stderr, "%s: invalid candidate (a=%lld b=%lld c=%lld k=%u j=%u which=%d)\n", kWho,
-                    (long long)cands[i].a, (long long)cands[i].b, (long long)cands[i].c, cands[i].k,
-                    cands[i].j, cands[i].which);
-            return -1;
-        }
-        geometry(digits_to_bits(dg), MIN_LOGM, &q.L, &q.logm);
-        if (q.logm > MAX_LOGM || q.L > MAX_LIMBS) {
-            fprintf(stderr, "%s: candidate of %lld digits is beyond this build's transform sizes\n",
-                    kWho, (long long)dg);
-            return -1;
-        }
-        memset(q.terms, 0, sizeof(q.terms));
-        for (size_t k = 0; k < t.size(); k++) {
-            q.terms[2 * k] = t[k].first;
-            q.terms[2 * k + 1] = t[k].second;
-        }
-        q.cand = cands[i];
-        q.tag = tags ? tags[i] : (int64_t)(S->reqs.size() + i);
-        q.rounds = 0;
-        q.t_admit = 0.0;
+    int64_t dg = 0;
+    if (!candidate_terms(cd, t, &dg) || (int)t.size() > TMAX) {
+        fprintf(stderr, "%s: invalid candidate (a=%lld b=%lld c=%lld k=%u j=%u which=%d)\n", kWho,
+                (long long)cd.a, (long long)cd.b, (long long)cd.c, cd.k, cd.j, cd.which);
+        return false;
     }
+    geometry(digits_to_bits(dg), MIN_LOGM, &q.L, &q.logm);
+    if (q.logm > MAX_LOGM || q.L > MAX_LIMBS) {
+        fprintf(stderr, "%s: candidate of %lld digits is beyond this build's transform sizes\n", kWho,
+                (long long)dg);
+        return false;
+    }
+    memset(q.terms, 0, sizeof(q.terms));
+    for (size_t k = 0; k < t.size(); k++) {
+        q.terms[2 * k] = t[k].first;
+        q.terms[2 * k + 1] = t[k].second;
+    }
+    q.cand = cd;
+    q.tag = tag;
+    q.rounds = 0;
+    q.t_admit = 0.0;
+    return true;
+}
+
+// Queue validated requests (pools created on demand).
+int enqueue(mr_sched *S, std::vector<Req> &add)
+{
     const double now = mono_now();
     if (hipSetDevice(S->device) != hipSuccess) return -2;
-    for (int i = 0; i < n; i++) {
-        Req &q = add[i];
+    for (Req &q : add) {
         Pool *&p = S->pools[q.logm];
         if (!p) {
             int rc = pool_new(S, q.logm, &p);
@@ -1985,9 +2038,137 @@ extern "C" int mr_sched_submit(mr_sched_t *S, const mr_candidate_t *cands, const
         }
         q.t_submit = now;
         p->pending.push_back((int)S->reqs.size());
-        S->reqs.push_back(q);
+        S->reqs.push_back(std::move(q));
     }
     return 0;
+}
+
+} // namespace
+
+extern "C" int mr_sched_submit(mr_sched_t *S, const mr_candidate_t *cands, const int64_t *tags, int n)
+{
+    if (!S || (!cands && n) || n < 0) return -1;
+    std::vector<Req> add((size_t)n);
+    for (int i = 0; i < n; i++)
+        if (!make_req(cands[i], tags ? tags[i] : (int64_t)(S->reqs.size() + i), add[i])) return -1;
+    return enqueue(S, add);
+}
+
+extern "C" int mr_sched_checkpoint(mr_sched_t *S, const char *path)
+{
+    if (!S || !path) return -1;
+    if (S->failed) return -2;
+    if (hipSetDevice(S->device) != hipSuccess) return -2;
+    // 1. let the work in flight finish and take its results
+    for (Pool *p : S->pools) {
+        if (!p) continue;
+        if (p->run_busy) {
+            hipError_t e = hipEventSynchronize(p->ev_run);
+            if (e != hipSuccess) { S->failed = true; return hip_fail(e, "checkpoint"); }
+            chunk_done(p);
+        }
+        if (p->adm_busy) {
+            hipError_t e = hipEventSynchronize(p->ev_adm);
+            if (e != hipSuccess) { S->failed = true; return hip_fail(e, "checkpoint"); }
+            if (admin_done(S, p) != 0) { S->failed = true; return -2; }
+        }
+    }
+    // 2. per undecided request: admitted (slot state) or still queued
+    struct Row { int r, bidx, bitpos; std::vector<uint32_t> res; };
+    std::vector<Row> rows;
+    for (Pool *p : S->pools) {
+        if (!p) continue;
+        std::vector<int32_t> bidx((size_t)p->n), bitpos((size_t)p->n);
+        MR_CHECK(hipMemcpy(bidx.data(), p->x->d.bidx, 4 * (size_t)p->n, hipMemcpyDeviceToHost));
+        MR_CHECK(hipMemcpy(bitpos.data(), p->x->d.bitpos, 4 * (size_t)p->n, hipMemcpyDeviceToHost));
+        for (int c = 0; c < p->n; c++) {
+            if (p->state[c] != SL_RUN && p->state[c] != SL_END) continue;
+            Row w{p->req[c], bidx[c], bitpos[c], {}};
+            w.res.resize((size_t)S->reqs[w.r].L);
+            MR_CHECK(hipMemcpy(w.res.data(), p->x->d.res + (size_t)c * p->x->stride, 4 * w.res.size(),
+                               hipMemcpyDeviceToHost));
+            rows.push_back(std::move(w));
+        }
+        for (size_t i = p->pend_head; i < p->pending.size(); i++) rows.push_back(Row{p->pending[i], -1, 0, {}});
+    }
+    std::sort(rows.begin(), rows.end(), [](const Row &x, const Row &y) { return x.r < y.r; });
+    // 3. the file (written whole, then renamed over path)
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE *f = fopen(tmp.c_str(), "w");
+    if (!f) return -1;
+    fprintf(f, "mr-sched-checkpoint 1\nbases %d", S->n_bases);
+    for (uint32_t b : S->bases) fprintf(f, " %u", b);
+    fprintf(f, "\ncandidates %zu\n", rows.size());
+    for (const Row &w : rows) {
+        const Req &q = S->reqs[w.r];
+        fprintf(f, "%lld %lld %lld %lld %u %u %d %d %d %d %zu", (long long)q.tag, (long long)q.cand.a,
+                (long long)q.cand.b, (long long)q.cand.c, q.cand.k, q.cand.j, q.cand.which, q.rounds,
+                w.bidx, w.bitpos, w.res.size());
+        for (uint32_t v : w.res) fprintf(f, " %x", v);
+        fputc('\n', f);
+    }
+    const bool ok = !ferror(f);
+    if (fclose(f) != 0 || !ok || rename(tmp.c_str(), path) != 0) {
+        (void)remove(tmp.c_str());
+        return -1;
+    }
+    return (int)rows.size();
+}
+
+extern "C" int mr_sched_restore(mr_sched_t *S, const char *path)
+{
+    if (!S || !path) return -1;
+    FILE *f = fopen(path, "r");
+    if (!f) return -1;
+    std::vector<Req> add;
+    auto bad = [&](const char *what) {
+        fprintf(stderr, "%s: checkpoint %s: %s\n", kWho, path, what);
+        fclose(f);
+        return -1;
+    };
+    int version = 0, nb = 0;
+    if (fscanf(f, "mr-sched-checkpoint %d bases %d", &version, &nb) != 2 || version != 1)
+        return bad("not a checkpoint of this format");
+    if (nb != S->n_bases) return bad("another list of bases");
+    for (int i = 0; i < nb; i++) {
+        unsigned b = 0;
+        if (fscanf(f, "%u", &b) != 1 || b != S->bases[i]) return bad("another list of bases");
+    }
+    long long count = 0;
+    if (fscanf(f, " candidates %lld", &count) != 1 || count < 0 || count > (1ll << 30))
+        return bad("bad candidate count");
+    add.resize((size_t)count);
+    for (long long i = 0; i < count; i++) {
+        long long tag, a, b, c;
+        unsigned k, j;
+        int which, rounds, bidx, bitpos;
+        size_t L;
+        if (fscanf(f, "%lld %lld %lld %lld %u %u %d %d %d %d %zu", &tag, &a, &b, &c, &k, &j, &which, &rounds,
+                   &bidx, &bitpos, &L) != 11)
+            return bad("truncated candidate line");
+        mr_candidate_t cd{};
+        cd.a = a; cd.b = b; cd.c = c; cd.k = k; cd.j = j; cd.which = which;
+        Req &q = add[(size_t)i];
+        if (!make_req(cd, tag, q)) return bad("invalid candidate");
+        if (rounds < 0 || rounds > nb || bidx < -1 || bidx >= nb) return bad("bad round state");
+        q.rounds = rounds;
+        if (bidx >= 0) {
+            if (L != (size_t)q.L || bitpos < -1 || bitpos > 24 * q.L) return bad("bad residue state");
+            q.resume_bidx = bidx;
+            q.resume_bitpos = bitpos;
+            q.resume_res.resize(L);
+            for (size_t w = 0; w < L; w++) {
+                unsigned v = 0;
+                if (fscanf(f, "%x", &v) != 1 || v > LIMB_MASK) return bad("bad residue");
+                q.resume_res[w] = v;
+            }
+        } else if (L != 0) {
+            return bad("residue on a queued candidate");
+        }
+    }
+    fclose(f);
+    const int rc = enqueue(S, add);
+    return rc ? rc : (int)count;
 }
 
 extern "C" int mr_sched_pump(mr_sched_t *S, int block)

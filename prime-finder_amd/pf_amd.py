@@ -237,6 +237,8 @@ def _mr_sigs(L):
         "mr_sched_results": (i32, [vp, C.POINTER(MrResult), i32]),
         "mr_sched_stats": (i32, [vp, C.POINTER(MrSchedStats)]),
         "mr_sched_destroy": (None, [vp]),
+        "mr_sched_checkpoint": (i32, [vp, C.c_char_p]),
+        "mr_sched_restore": (i32, [vp, C.c_char_p]),
         "mr_test_batch": (i32, [C.POINTER(MrCandidate), i32, C.POINTER(u32), i32, i32,
                                 C.POINTER(C.c_uint8), C.POINTER(C.c_int32)]),
     }
@@ -316,6 +318,21 @@ class MrScheduler:
         if self.L.mr_sched_stats(self.ptr, C.byref(st)) != 0:
             raise RuntimeError("mr_sched_stats failed")
         return st
+
+    def checkpoint(self, path) -> int:
+        """Write the undecided candidates (and the progress of admitted
+        ones) to path; returns how many."""
+        n = self.L.mr_sched_checkpoint(self.ptr, os.fsencode(path))
+        if n < 0:
+            raise RuntimeError(f"mr_sched_checkpoint failed ({n})")
+        return n
+
+    def restore(self, path) -> int:
+        """Queue the candidates of a checkpoint file; returns how many."""
+        n = self.L.mr_sched_restore(self.ptr, os.fsencode(path))
+        if n < 0:
+            raise ValueError(f"mr_sched_restore rejected {path}")
+        return n
 
     def run_all(self):
         res = []

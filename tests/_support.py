@@ -209,6 +209,19 @@ def gmp_version() -> str:
     return _mr_oracle().mr_oracle_gmp_version().decode()
 
 
+def gmp_powm(base: int, e: int, N: int) -> int:
+    """base^e mod N with GMP mpz_powm (the oracle's harness)."""
+    nw = (N.bit_length() + 31) // 32
+    ne = max(1, (e.bit_length() + 31) // 32)
+    out = np.zeros(nw, np.uint32)
+    Nw, Bw, Ew = int_words(N, nw), int_words(base % N, nw), int_words(e, ne)
+    rc = _mr_oracle().mr_oracle_powm(Nw.ctypes.data_as(u32p), nw, Bw.ctypes.data_as(u32p),
+                                     Ew.ctypes.data_as(u32p), ne, out.ctypes.data_as(u32p))
+    if rc != 0:
+        raise ValueError("mr_oracle_powm failed")
+    return words_int(out)
+
+
 def inv_pow2_mod(n: int, R: int) -> int:
     """R^-1 mod n for odd n and R a power of two: u = n^-1 mod R by Newton
     (y <- y (2 - n y), doubling the bits), n u = 1 + k R, so R (-k) == 1

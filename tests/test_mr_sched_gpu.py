@@ -114,3 +114,90 @@ def test_invalid_submission_rejected(gpu):
         s.submit([(200000, 5, 1, 1, 1, 0)])        # beyond 2^15 transforms
     assert s.stats().submitted == 0
     s.close()
+
+
+# 600-digit primes of the candidate form (found with GMP, re-checked below)
+PRIMES_600 = [(600, 475, 249, 1179, 1874, 0), (600, 360, 167, 77, 87, 0),
+              (600, 410, 392, 2792, 3950, 0)]
+
+
+def read_checkpoint(path):
+    """Rows (tag, candidate, rounds, base index, exponent bit, residue) of a
+    checkpoint file, and its bases."""
+    with open(path) as f:
+        head = f.readline().split()
+        assert head == ["mr-sched-checkpoint", "1"]
+        b = f.readline().split()
+        bases = [int(x) for x in b[2:]]
+        assert b[0] == "bases" and int(b[1]) == len(bases)
+        cnt = f.readline().split()
+        rows = []
+        for line in f:
+            v = line.split()
+            tag, a, bb, c, k, j, w, rounds, bidx, bitpos, L = (int(x) for x in v[:11])
+            res = [int(x, 16) for x in v[11:]]
+            assert len(res) == L
+            rows.append((tag, (a, bb, c, k, j, w), rounds, bidx, bitpos, L, res))
+        assert cnt[0] == "candidates" and int(cnt[1]) == len(rows)
+    return bases, rows
+
+
+def test_checkpoint_and_resume(gpu, tmp_path):
+    """Checkpoint a running scheduler mid-exponent, restore it into a fresh
+    one and finish: every saved residue equals base^(N >> (bit + 1)) R mod N
+    (GMP), and verdicts and round counts equal an uninterrupted run's
+    (known primes included, so a damaged residue cannot pass as composite)."""
+    from _support import gmp_powm, gmp_probab_prime
+    for cd in PRIMES_600:
+        assert gmp_probab_prime(gmp_build(*cd), 25)
+    rng = random.Random(77)
+    cands = PRIMES_600 + mixed_candidates(rng, 600, 5, which=(0, 1)) + \
+        mixed_candidates(rng, 7500, 4, which=(0, 1))
+    ref_v, ref_r = gpu.mr_test_batch(cands, BASES)
+    assert [ref_v[i] for i in range(3)] == [1, 1, 1]
+    s1 = gpu.MrScheduler(BASES, chunk=32)
+    s1.submit(cands, tags=list(range(len(cands))))
+    for _ in range(6):
+        s1.pump(True)
+    path = tmp_path / "sched.ckpt"
+    n = s1.checkpoint(path)
+    before = {r.tag: (r.verdict, r.rounds) for r in s1.results()}
+    s1.close()
+    say("checkpoint:", n, "undecided,", len(before), "decided before")
+    assert n == len(cands) - len(before) and n > 0
+    bases, rows = read_checkpoint(path)
+    assert bases == BASES and len(rows) == n
+    mid = 0
+    for tag, cd, rounds, bidx, bitpos, L, res in rows:
+        assert cd == tuple(cands[tag])
+        if bidx < 0:
+            continue
+        N = gmp_build(*cd)
+        R = 1 << (24 * L)
+        r = sum(v << (24 * i) for i, v in enumerate(res))
+        assert r == gmp_powm(BASES[bidx], N >> (bitpos + 1), N) * R % N, f"residue of {cd}"
+        s = ((N - 1) & -(N - 1)).bit_length() - 1
+        mid += s <= bitpos < N.bit_length() - 2
+    say("checkpoint:", sum(r[3] >= 0 for r in rows), "admitted,", mid, "mid-exponent")
+    assert mid >= 1
+    s2 = gpu.MrScheduler(BASES, chunk=32)
+    assert s2.restore(path) == n
+    after = {r.tag: (r.verdict, r.rounds) for r in s2.run_all()}
+    s2.close()
+    assert set(before) | set(after) == set(range(len(cands))) and not set(before) & set(after)
+    got = {**before, **after}
+    for i in range(len(cands)):
+        assert got[i] == (ref_v[i], ref_r[i]), f"candidate {cands[i]}"
+    # refusals: other bases, a truncated file, no file
+    s3 = gpu.MrScheduler([2, 3])
+    with pytest.raises(ValueError):
+        s3.restore(path)
+    s3.close()
+    cut = tmp_path / "cut.ckpt"
+    cut.write_text("\n".join(path.read_text().split("\n")[:3]) + "\n")      # no rows
+    s4 = gpu.MrScheduler(BASES)
+    with pytest.raises(ValueError):
+        s4.restore(cut)
+    with pytest.raises(ValueError):
+        s4.restore(tmp_path / "missing.ckpt")
+    s4.close()
