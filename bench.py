@@ -129,6 +129,35 @@ def mr_leg(pf, n_bits: int) -> dict:
     return res
 
 
+def mr_cpu_baseline(cands, budget_s: float = 15.0, sample_bits: int = 4096) -> dict:
+    """GMP mpz_powm (the oracle's harness, oracle/mr_oracle.c) per (bigint,
+    base 2) on one core: candidates of the workload in turn, each over the
+    top `sample_bits` bits of its d, until about budget_s seconds, scaled
+    to the whole exponent."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from _support import gmp_build, gmp_powm
+    spent, scaled, done, bits = 0.0, 0.0, 0, 0
+    for cd in cands:
+        n = gmp_build(*cd)
+        d = n - 1
+        while d % 2 == 0:
+            d //= 2
+        bits = d.bit_length()
+        e = d >> max(0, bits - sample_bits)
+        t0 = time.perf_counter()
+        gmp_powm(2, e, n)
+        dt = time.perf_counter() - t0
+        spent += dt
+        scaled += dt * bits / sample_bits
+        done += 1
+        if spent >= budget_s:
+            break
+    return {"value": 3600.0 * done / scaled, "unit": "tests/hour", "cores": 1, "kind": "port",
+            "sample": (f"{done} candidates of the workload, GMP mpz_powm of base 2 over the top "
+                       f"{sample_bits} of {bits} bits of d, scaled to the whole exponent "
+                       f"(extrapolated); {spent:.1f} s on 1 core")}
+
+
 def workload3(pf, args) -> dict:
     """Workload 3 on its own line: tests/hour of the batched path at 50 000
     digits, extrapolated from the timed squaring steps (every candidate of
@@ -153,6 +182,8 @@ def workload3(pf, args) -> dict:
         "plain_one_candidate": {"us_per_squaring": one_step_s * 1e6,
                                 "tests_per_hour": plain_tph,
                                 "batched_over_plain": leg["tests_per_hour_extrapolated"] / plain_tph},
+        "cpu_baseline": (mr_cpu_baseline([tuple(x) + (w,) for x in cfg["candidates"] for w in (0, 1)])
+                         if not args.no_cpu_baseline else None),
         "device": pf.device_info(),
     }
 

@@ -39,6 +39,8 @@ struct Dev {                    // per-batch device state (kernel argument)
     int stride;                 // words per linear limb vector (>= L + 1)
     int n;                      // candidate slots
     const int32_t *list;        // launch index -> slot (nullptr: identity)
+    const uint8_t *run;         // [n] scheduler graph: 1 = the slot squares in this chunk
+                                //   (nullptr: every launched slot)
     int32_t *Ls;                // [n] limbs of the slot's number (R = 2^(24 Ls))
     uint32_t *base;             // [n] base of the current round
     int32_t *bidx;              // [n] index of that base in the scheduler's list
@@ -881,6 +883,7 @@ __global__ __launch_bounds__(THREADS) void exp_kernel(Dev D, Twiddles T, Consts 
     constexpr int C = G<LOGM>::C;
     __shared__ Smem<LOGM> sm;
     const int c = slot_id(D);
+    if (D.run && !D.run[c]) return;      // (slots under admission are not touched)
     if (!D.alive[c] || D.pre[c]) return;
     const int s = D.sval[c];
     int bp = D.bitpos[c];
@@ -1688,14 +1691,18 @@ struct Pool {
     mr_ctx *x = nullptr;            // x->st is the admin stream
     hipStream_t st_run = nullptr;
     hipEvent_t ev_run = nullptr, ev_adm = nullptr;
-    int32_t *d_list_run = nullptr, *d_list_adm = nullptr, *d_Ls = nullptr;
+    int32_t *d_list_adm = nullptr, *d_Ls = nullptr;
     int64_t *d_terms = nullptr;
     uint32_t *d_bases = nullptr;
     int32_t *d_resume = nullptr;    // [n][3] slot, base index, exponent bit
+    uint8_t *d_run = nullptr;       // [n] slots that square in the next chunk
+    uint8_t *h_run = nullptr;
+    hipGraph_t graph = nullptr;     // one chunk: run flags in, squarings, exponent bits out
+    hipGraphExec_t gexec = nullptr;
     int32_t *h_resume = nullptr;
     uint32_t *h_resres = nullptr;   // [n][stride] residues to resume (pinned, on first use)
     // pinned host mirrors
-    int32_t *h_list_run = nullptr, *h_list_adm = nullptr, *h_Ls = nullptr;
+    int32_t *h_list_adm = nullptr, *h_Ls = nullptr;
     int32_t *h_bitpos = nullptr, *h_sval = nullptr;
     int64_t *h_terms = nullptr;
     uint8_t *h_alive = nullptr, *h_done = nullptr;
@@ -1731,11 +1738,13 @@ void pool_free(Pool *p)
     if (!p) return;
     if (p->st_run) (void)hipStreamSynchronize(p->st_run);
     if (p->x && p->x->st) (void)hipStreamSynchronize(p->x->st);
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
     if (p->ev_run) (void)hipEventDestroy(p->ev_run);
     if (p->ev_adm) (void)hipEventDestroy(p->ev_adm);
     if (p->st_run) (void)hipStreamDestroy(p->st_run);
-    void *hp[] = {p->h_list_run, p->h_list_adm, p->h_Ls, p->h_bitpos, p->h_sval,
-                  p->h_terms, p->h_alive, p->h_done, p->h_err, p->h_resume, p->h_resres};
+    void *hp[] = {p->h_list_adm, p->h_Ls, p->h_bitpos, p->h_sval,
+                  p->h_terms, p->h_alive, p->h_done, p->h_err, p->h_resume, p->h_resres, p->h_run};
     for (void *q : hp) if (q) (void)hipHostFree(q);
     if (p->x) mr_ctx_destroy(p->x);       // frees the device arrays it owns
     delete p;
@@ -1763,16 +1772,15 @@ int pool_new(mr_sched *S, int logm, Pool **out)
     if (rc == 0) rc = ctx_alloc_all(p->x, 24l * lmax - 1, logm);
     if (rc == 0 && p->x->logm != logm) rc = -1;
     auto dev = [&](void **q, size_t b) { if (rc == 0 && ctx_alloc(p->x, q, b) != 0) rc = -2; };
-    dev((void **)&p->d_list_run, 4 * (size_t)n);
     dev((void **)&p->d_list_adm, 4 * (size_t)n);
     dev((void **)&p->d_Ls, 4 * (size_t)n);
     dev((void **)&p->d_terms, sizeof(int64_t) * TMAX * 2 * (size_t)n);
     dev((void **)&p->d_bases, 4 * (size_t)S->n_bases);
     dev((void **)&p->d_resume, 12 * (size_t)n);
+    dev((void **)&p->d_run, (size_t)n);
     auto host = [&](void **q, size_t b) {
         if (rc == 0 && hipHostMalloc(q, b ? b : 4, hipHostMallocDefault) != hipSuccess) rc = -2;
     };
-    host((void **)&p->h_list_run, 4 * (size_t)n);
     host((void **)&p->h_list_adm, 4 * (size_t)n);
     host((void **)&p->h_Ls, 4 * (size_t)n);
     host((void **)&p->h_bitpos, 4 * (size_t)n);
@@ -1782,6 +1790,7 @@ int pool_new(mr_sched *S, int logm, Pool **out)
     host((void **)&p->h_done, (size_t)n);
     host((void **)&p->h_err, 4);
     host((void **)&p->h_resume, 12 * (size_t)n);
+    host((void **)&p->h_run, (size_t)n);
     if (rc == 0 && (hipStreamCreateWithFlags(&p->st_run, hipStreamNonBlocking) != hipSuccess ||
                     hipEventCreateWithFlags(&p->ev_run, hipEventDisableTiming) != hipSuccess ||
                     hipEventCreateWithFlags(&p->ev_adm, hipEventDisableTiming) != hipSuccess))
@@ -1815,28 +1824,53 @@ void finish(mr_sched *S, int r, int verdict)
     S->n_done++;
 }
 
-// Launch one squaring chunk over the pool's RUN slots.
+// The pool's chunk as a HIP graph, captured once: copy the run flags in,
+// one squaring kernel over every slot of the pool (slots not running return
+// at once), copy the exponent bits out.  Fixed pointers and chunk length,
+// so each chunk is one hipGraphLaunch; no memset nodes.
+int pool_graph(mr_sched *S, Pool *p)
+{
+    hipStream_t st = p->st_run;
+    Dev Dg = p->x->d;
+    Dg.list = nullptr;
+    Dg.run = p->d_run;
+    const int chunk = S->chunk, n = p->n;
+    MR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    hipError_t e = hipMemcpyAsync(p->d_run, p->h_run, (size_t)n, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) {
+        (void)dispatch_logm(p->logm, [&](auto lg) {
+            constexpr int LG = decltype(lg)::value;
+            exp_kernel<LG><<<n, THREADS, 0, st>>>(Dg, p->x->T, p->x->K, chunk);
+            return 0;
+        });
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(p->h_bitpos, p->x->d.bitpos, 4 * (size_t)n, hipMemcpyDeviceToHost, st);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(st, &g);
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&p->gexec, g, nullptr, nullptr, 0);
+    p->graph = g;
+    if (e != hipSuccess) { p->gexec = nullptr; return hip_fail(e, "chunk graph"); }
+    return 0;
+}
+
+// Launch one squaring chunk over the pool's RUN slots (one graph launch).
 int launch_chunk(mr_sched *S, Pool *p)
 {
     p->run_list.clear();
-    for (int c = 0; c < p->n; c++)
-        if (p->state[c] == SL_RUN) p->run_list.push_back(c);
+    for (int c = 0; c < p->n; c++) {
+        const bool run = p->state[c] == SL_RUN;
+        p->h_run[c] = run ? 1 : 0;
+        if (run) p->run_list.push_back(c);
+    }
     if (p->run_list.empty()) return 0;
     const int nr = (int)p->run_list.size();
-    memcpy(p->h_list_run, p->run_list.data(), 4 * (size_t)nr);
-    MR_CHECK(hipMemcpyAsync(p->d_list_run, p->h_list_run, 4 * (size_t)nr, hipMemcpyHostToDevice, p->st_run));
-    Dev Dl = p->x->d;
-    Dl.list = p->d_list_run;
-    const int chunk = S->chunk;
-    int rc = dispatch_logm(p->logm, [&](auto lg) {
-        constexpr int LG = decltype(lg)::value;
-        exp_kernel<LG><<<nr, THREADS, 0, p->st_run>>>(Dl, p->x->T, p->x->K, chunk);
-        return 0;
-    });
-    if (rc) return rc;
-    MR_CHECK(hipGetLastError());
-    MR_CHECK(hipMemcpyAsync(p->h_bitpos, p->x->d.bitpos, 4 * (size_t)p->n, hipMemcpyDeviceToHost, p->st_run));
+    if (!p->gexec) { int rc = pool_graph(S, p); if (rc) return rc; }
+    MR_CHECK(hipGraphLaunch(p->gexec, p->st_run));
     MR_CHECK(hipEventRecord(p->ev_run, p->st_run));
+    const int chunk = S->chunk;
     p->run_busy = true;
     S->chunks++;
     S->squarings += (int64_t)nr * chunk;
