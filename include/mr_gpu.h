@@ -85,7 +85,8 @@ void mr_ctx_destroy(mr_ctx_t *ctx);
  * of bases, retiring each at its first failing base.  Candidates are sorted
  * into size buckets by transform length (2^11 .. 2^15 points); each bucket
  * owns a pool of slots in HBM (one workgroup each).  Squarings run in chunks
- * of `chunk` steps over every running slot on one stream, while a second
+ * of `chunk` steps over every running slot on one stream (one HIP graph
+ * launch per chunk), while a second
  * stream ends rounds, starts the next base and builds newly admitted
  * candidates into freed slots, so the GPU never waits for admission.  At
  * most `max_active` slots (0: the GPU's compute-unit count) are in use at
@@ -148,7 +149,8 @@ void mr_sched_destroy(mr_sched_t *s);
  * mr_sched_restore queues the candidates of such a file on s, which must
  * have been created with the same list of bases; admitted ones resume at
  * their saved exponent bit.  Returns the number restored, or -1 (unreadable
- * or malformed file, other bases, invalid candidate; nothing queued). */
+ * or malformed file, other bases, invalid candidate; nothing queued), -2
+ * (HIP error or out of memory while queueing). */
 int mr_sched_checkpoint(mr_sched_t *s, const char *path);
 int mr_sched_restore(mr_sched_t *s, const char *path);
 

@@ -2088,9 +2088,10 @@ extern "C" int mr_sched_submit(mr_sched_t *S, const mr_candidate_t *cands, const
     return enqueue(S, add);
 }
 
-extern "C" int mr_sched_checkpoint(mr_sched_t *S, const char *path)
+namespace {
+
+int sched_checkpoint(mr_sched *S, const char *path)
 {
-    if (!S || !path) return -1;
     if (S->failed) return -2;
     if (hipSetDevice(S->device) != hipSuccess) return -2;
     // 1. let the work in flight finish and take its results
@@ -2149,9 +2150,22 @@ extern "C" int mr_sched_checkpoint(mr_sched_t *S, const char *path)
     return (int)rows.size();
 }
 
-extern "C" int mr_sched_restore(mr_sched_t *S, const char *path)
+} // namespace
+
+extern "C" int mr_sched_checkpoint(mr_sched_t *S, const char *path)
 {
     if (!S || !path) return -1;
+    try {
+        return sched_checkpoint(S, path);
+    } catch (...) {             // allocation failure: nothing escapes the C ABI
+        return -2;
+    }
+}
+
+namespace {
+
+int sched_restore(mr_sched *S, const char *path)
+{
     FILE *f = fopen(path, "r");
     if (!f) return -1;
     std::vector<Req> add;
@@ -2171,7 +2185,7 @@ extern "C" int mr_sched_restore(mr_sched_t *S, const char *path)
     long long count = 0;
     if (fscanf(f, " candidates %lld", &count) != 1 || count < 0 || count > (1ll << 30))
         return bad("bad candidate count");
-    add.resize((size_t)count);
+    add.reserve((size_t)std::min(count, 1ll << 16));   // the lines, not the header, set the size
     for (long long i = 0; i < count; i++) {
         long long tag, a, b, c;
         unsigned k, j;
@@ -2182,7 +2196,8 @@ extern "C" int mr_sched_restore(mr_sched_t *S, const char *path)
             return bad("truncated candidate line");
         mr_candidate_t cd{};
         cd.a = a; cd.b = b; cd.c = c; cd.k = k; cd.j = j; cd.which = which;
-        Req &q = add[(size_t)i];
+        add.emplace_back();
+        Req &q = add.back();
         if (!make_req(cd, tag, q)) return bad("invalid candidate");
         if (rounds < 0 || rounds > nb || bidx < -1 || bidx >= nb) return bad("bad round state");
         q.rounds = rounds;
@@ -2203,6 +2218,18 @@ extern "C" int mr_sched_restore(mr_sched_t *S, const char *path)
     fclose(f);
     const int rc = enqueue(S, add);
     return rc ? rc : (int)count;
+}
+
+} // namespace
+
+extern "C" int mr_sched_restore(mr_sched_t *S, const char *path)
+{
+    if (!S || !path) return -1;
+    try {
+        return sched_restore(S, path);
+    } catch (...) {             // allocation failure: nothing escapes the C ABI
+        return -2;
+    }
 }
 
 extern "C" int mr_sched_pump(mr_sched_t *S, int block)
