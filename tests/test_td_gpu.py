@@ -110,6 +110,23 @@ def test_4096_sampled_primes_vs_oracle(gpu):
         assert (got == ref).all(), (a, b, c)
 
 
+def test_max_grid_sampled_primes_vs_oracle(gpu):
+    """K = J = 9999 (the largest grid): the lane kernel's fixed-point
+    residue images carry the most accumulated rounding there (up to K + 1
+    steps), so its candidate windows are widest; sampled primes from 2^16
+    to 10^8 and next to 2^31 against the oracle, bit for bit."""
+    allp = gpu.primes_below(10 ** 8)
+    big = allp[allp >= 65536][::20011]
+    extra = np.array([65537, 65539, 2147483647, 2147483629], np.uint32)
+    primes = np.concatenate([allp[(allp > 60000) & (allp < 65536)][:8], big, extra])
+    for (a, b, c) in [(60000, 31234, 100), (50001, 33334, 2749)]:
+        say("max grid 9999", a, b, c, len(primes))
+        rc, got = gpu.trial_div_gpu_kj(primes, a, b, c, 9999, 9999)
+        assert rc == 0
+        ref = oracle_sieve(primes, a, b, c, 9999, 9999)
+        assert (got == ref).all(), (a, b, c)
+
+
 def test_workload2_full_triple_properties(gpu):
     """One full workload-2 triple (5 761 455 primes, 4096 x 4096): sampled
     bits against full-list oracle verdicts, determinism, and the OR-split
