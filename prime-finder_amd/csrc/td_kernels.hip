@@ -397,6 +397,15 @@ __device__ __forceinline__ void td_gmark(uint32_t *bits, uint32_t bit)
 // names (lj, rev(j)) uniquely; rdec[R] holds that j (td_rdec_kernel).
 static_assert(TD_SMALL_LIMIT > 9999u, "lane kernel assumes one j per residue");
 
+// Lanes with p >= TD_FIX_MIN track fixed-point images of their residues
+// (1 op per residue and k, candidates recomputed exactly).  Smaller primes
+// have candidates at more k (a wave of 64 lanes enters the candidate path
+// with probability ~64 x 14 095 / p per k), where exact residues (3 ops per
+// residue and k, hits read directly) cost less.  Measured per triple at
+// K = J = 4096: 4.46 ms with the switch at 2^20, 4.30 at 2^22, 4.23 at
+// 2^23, 4.20 at 2^24; 5.17 with exact residues everywhere.
+constexpr uint32_t TD_FIX_MIN = 1u << 24;
+
 template <int DJ, bool MERGED, int NR>
 __device__ __forceinline__ void td_hits(uint32_t k, uint32_t un, uint32_t thrn,
                                         const uint32_t (&v)[NR], const uint32_t (&thr)[NR],
@@ -419,6 +428,48 @@ __device__ __forceinline__ void td_hits(uint32_t k, uint32_t un, uint32_t thrn,
         }
     }
 }
+
+// Fixed-point image of a residue x mod p (x < p): round(x 2^32 / p), with
+// fp = 2^32 / p in double precision (relative error 2^-52, far below one
+// unit).  The image of (x - s) mod p is the image of x minus the image of s
+// modulo 2^32, up to one rounding error per subtraction, so a residue is
+// advanced with one subtraction (the wrap mod 2^32 is the wrap mod p) and
+// tested with one compare against its window widened by the error
+// accumulated so far; the rare candidates are recomputed exactly.
+__device__ __forceinline__ uint32_t td_fix(uint32_t x, double fp)
+{
+    return (uint32_t)__double2ull_rn((double)x * fp);
+}
+
+// Upper end of the candidate window of residues v < thr, images shifted by
+// eoff (the error bound): true hits have images in [0, thr fp + 2 eoff).
+__device__ __forceinline__ uint32_t td_fix_window(uint32_t thr, double fp, uint32_t eoff)
+{
+    return thr ? (uint32_t)ceil((double)thr * fp) + 2u * eoff : 0u;
+}
+
+// Candidate of the fixed-point test: the exact residues of row k, then the
+// marks of td_hits.  un(k) = un0 - k bstep, v_l(k) = cm_l - rev(k) E_l
+// (mod p), rev(k) with lk digits; bstepM and eM_l in Montgomery form.
+template <int DJ, bool MERGED, int NR>
+__device__ __forceinline__ void td_hits_exact(uint32_t k, int lk, uint32_t un0, uint32_t bstepM,
+                                              const uint32_t (&cm)[NR], const uint32_t (&eM)[NR],
+                                              const uint32_t (&thr)[NR], PfMont m,
+                                              uint32_t J, const uint16_t *__restrict__ rdec,
+                                              uint32_t *bits)
+{
+    const uint32_t p = m.p;
+    const uint32_t un = pf_submod(un0, pf_mont_mul(bstepM, k, m), p);
+    const uint32_t rk = td_reverse(k, lk);
+    uint32_t v[NR];
+#pragma unroll
+    for (int l = 0; l < NR; l++) v[l] = pf_submod(cm[l], pf_mont_mul(eM[l], rk, m), p);
+    td_hits<DJ, MERGED, NR>(k, un, J, v, thr, J, rdec, bits);
+}
+
+// True when some lane of the wave has h set: a wave-uniform branch (one
+// scalar test), so the loop does not save and restore the exec mask per k.
+__device__ __forceinline__ bool td_any(bool h) { return __ballot(h) != 0ull; }
 
 // rdec tables for D = 1..4 at offsets 0, 10, 110, 1110: rdec_D[R] = the j
 // whose rev(j) padded with zeros to D digits is R (0 for R = 0).
@@ -470,9 +521,7 @@ void td_large_kernel(const uint32_t *__restrict__ primes, uint32_t n,
     const uint32_t icS = pf_mont_out(pf_mont_pow(tenInv, (uint64_t)c, m), m);   // 10^-c
     const uint32_t ra1M = pf_submod(raM, m.one, p);                              // ra - 1
     const uint32_t bstep = pf_mont_mul(rbM, icS, m);                             // rb / rc
-    uint32_t un = pf_submod(pf_mont_mul(ra1M, icS, m), 1u, p);                   // u at k = 0
-    const uint32_t thrn = J;
-
+    const uint32_t un0 = pf_submod(pf_mont_mul(ra1M, icS, m), 1u, p);            // u at k = 0
     // Rev residue l is (ra - 1 - rev(k) kres) / X_l - 1 with
     //   separate: X_l = 10^(a-c-(l+1)), hit range [0, 10^(l+1) - 1);
     //   merged:   X_0 = 10^(a-c-DJ),    hit range [0, 10^DJ - 1).
@@ -493,29 +542,118 @@ void td_large_kernel(const uint32_t *__restrict__ primes, uint32_t n,
     const long long eb1 = a - b - 1;
     uint32_t kresM = eb1 >= 0 ? pf_mont_pow(ten, (uint64_t)eb1, m) : 0u;
 
+    if (p < TD_FIX_MIN) {
+        // exact residues: 3 ops per residue and k, the hit path has them at hand
+        uint32_t un = un0;
+        const uint32_t thrn = J;
+        const int DK = (K >= 1000u) ? 4 : (K >= 100u) ? 3 : (K >= 10u) ? 2 : 1;
+        for (int lk = 1; lk <= DK; lk++) {
+            const uint32_t klo = td_pow10(lk - 1);
+            const uint32_t khi = min(K, td_pow10(lk) - 1u);
+            const bool kok = (a - b - lk) >= 0;
+            // Steps of rev(k) inside a row of ten (units digit of k -> top digit
+            // of rev(k)) and between rows (q -> q+1 with t9 trailing nines).
+            const uint32_t s0w = td_pow10(lk - 1);
+            uint32_t thr[NR], s0[NR], w[NR], tq0[NR], tq1[NR], tq2[NR];
+    #pragma unroll
+            for (int l = 0; l < NR; l++) {
+                const uint32_t eM = pf_mont_mul(kresM, jinvM[l], m);      // E = kres / X_l
+                thr[l] = (kok && jok[l]) ? td_pow10(MERGED ? DJ : l + 1) - 1u : 0u;
+                s0[l] = pf_mont_mul(eM, s0w, m);
+                const uint32_t eS = pf_mont_out(eM, m);
+                w[l] = lk == 1 ? cm[l] : pf_submod(cm[l], eS, p);         // rev(10^(lk-1)) = 1
+                // delta_t = 10^(lk-2-t) - 9 * sum_{i<t} 10^(lk-2-i), reduced mod p.
+                int d0 = lk >= 2 ? (int)td_pow10(lk - 2) : 0;
+                int d1 = lk >= 3 ? (int)td_pow10(lk - 3) - 9 * (int)td_pow10(lk - 2) : 0;
+                int d2 = lk >= 4 ? (int)td_pow10(lk - 4) - 9 * ((int)td_pow10(lk - 2) + (int)td_pow10(lk - 3)) : 0;
+                tq0[l] = pf_mont_mul(eM, d0 >= 0 ? (uint32_t)d0 : p - (uint32_t)(-d0), m);
+                tq1[l] = pf_mont_mul(eM, d1 >= 0 ? (uint32_t)d1 : p - (uint32_t)(-d1), m);
+                tq2[l] = pf_mont_mul(eM, d2 >= 0 ? (uint32_t)d2 : p - (uint32_t)(-d2), m);
+            }
+
+            const uint32_t qlo = klo / 10u, qhi = khi / 10u;
+            for (uint32_t q = qlo; q <= qhi; q++) {
+                const uint32_t kb = q * 10u;
+                const uint32_t dlo = kb < klo ? klo - kb : 0u;
+                const uint32_t dhi = min(9u, khi - kb);
+                uint32_t v[NR];
+    #pragma unroll
+                for (int l = 0; l < NR; l++) v[l] = w[l];
+                if (dlo == 0u && dhi == 9u) {
+    #pragma unroll
+                    for (uint32_t d = 0; d < 10u; d++) {
+                        un = pf_submod(un, bstep, p);
+                        bool h = un < thrn;
+    #pragma unroll
+                        for (int l = 0; l < NR; l++) h |= v[l] < thr[l];
+                        if (__builtin_expect(td_any(h), 0))
+                            if (h) td_hits<DJ, MERGED, NR>(kb + d, un, thrn, v, thr, J, rdec, bits);
+    #pragma unroll
+                        for (int l = 0; l < NR; l++) v[l] = pf_submod(v[l], s0[l], p);
+                    }
+                } else {
+                    for (uint32_t d = 0; d < dlo; d++) {
+    #pragma unroll
+                        for (int l = 0; l < NR; l++) v[l] = pf_submod(v[l], s0[l], p);
+                    }
+                    for (uint32_t d = dlo; d <= dhi; d++) {
+                        un = pf_submod(un, bstep, p);
+                        bool h = un < thrn;
+    #pragma unroll
+                        for (int l = 0; l < NR; l++) h |= v[l] < thr[l];
+                        if (h) td_hits<DJ, MERGED, NR>(kb + d, un, thrn, v, thr, J, rdec, bits);
+    #pragma unroll
+                        for (int l = 0; l < NR; l++) v[l] = pf_submod(v[l], s0[l], p);
+                    }
+                }
+                if (q < qhi) {
+                    uint32_t t9 = 0, qq = q;
+                    while (qq % 10u == 9u) { t9++; qq /= 10u; }
+    #pragma unroll
+                    for (int l = 0; l < NR; l++)
+                        w[l] = pf_submod(w[l], t9 == 0 ? tq0[l] : t9 == 1 ? tq1[l] : tq2[l], p);
+                }
+            }
+            kresM = pf_mont_mul(kresM, tenInv, m);
+        }
+        return;
+    }
+    // fixed-point images: 1 op per residue and k, candidates recomputed exactly
+    // Fixed-point images (td_fix).  Every image below is derived from an
+    // exact residue once per prime (un) or once per digit count of k (the
+    // rev residues) and then advanced at most K + 1 times, half a unit of
+    // error each, so eoff = 2K + 32 bounds the error.
+    const double fp = 4294967296.0 / (double)p;
+    const uint32_t eoff = 2u * K + 32u;
+    const uint32_t bF = td_fix(bstep, fp);
+    const uint32_t tU = td_fix_window(J, fp, eoff);
+    const uint32_t bstepM = pf_mont_in(bstep, m);
+    uint32_t uF = td_fix(un0, fp) + eoff;
     const int DK = (K >= 1000u) ? 4 : (K >= 100u) ? 3 : (K >= 10u) ? 2 : 1;
     for (int lk = 1; lk <= DK; lk++) {
         const uint32_t klo = td_pow10(lk - 1);
         const uint32_t khi = min(K, td_pow10(lk) - 1u);
         const bool kok = (a - b - lk) >= 0;
-        // Steps of rev(k) inside a row of ten (units digit of k -> top digit
-        // of rev(k)) and between rows (q -> q+1 with t9 trailing nines).
         const uint32_t s0w = td_pow10(lk - 1);
-        uint32_t thr[NR], s0[NR], w[NR], tq0[NR], tq1[NR], tq2[NR];
+        // exact per class (for the candidates' recheck): thr, E_l; images of
+        // the row start w, the in-row step s0 and the row-to-row steps tq
+        uint32_t thr[NR], eMl[NR], tV[NR], sF[NR], wF[NR], q0F[NR], q1F[NR], q2F[NR];
 #pragma unroll
         for (int l = 0; l < NR; l++) {
             const uint32_t eM = pf_mont_mul(kresM, jinvM[l], m);      // E = kres / X_l
+            eMl[l] = eM;
             thr[l] = (kok && jok[l]) ? td_pow10(MERGED ? DJ : l + 1) - 1u : 0u;
-            s0[l] = pf_mont_mul(eM, s0w, m);
+            tV[l] = td_fix_window(thr[l], fp, eoff);
+            sF[l] = td_fix(pf_mont_mul(eM, s0w, m), fp);
             const uint32_t eS = pf_mont_out(eM, m);
-            w[l] = lk == 1 ? cm[l] : pf_submod(cm[l], eS, p);         // rev(10^(lk-1)) = 1
+            wF[l] = td_fix(lk == 1 ? cm[l] : pf_submod(cm[l], eS, p), fp) + eoff;   // rev(10^(lk-1)) = 1
             // delta_t = 10^(lk-2-t) - 9 * sum_{i<t} 10^(lk-2-i), reduced mod p.
             int d0 = lk >= 2 ? (int)td_pow10(lk - 2) : 0;
             int d1 = lk >= 3 ? (int)td_pow10(lk - 3) - 9 * (int)td_pow10(lk - 2) : 0;
             int d2 = lk >= 4 ? (int)td_pow10(lk - 4) - 9 * ((int)td_pow10(lk - 2) + (int)td_pow10(lk - 3)) : 0;
-            tq0[l] = pf_mont_mul(eM, d0 >= 0 ? (uint32_t)d0 : p - (uint32_t)(-d0), m);
-            tq1[l] = pf_mont_mul(eM, d1 >= 0 ? (uint32_t)d1 : p - (uint32_t)(-d1), m);
-            tq2[l] = pf_mont_mul(eM, d2 >= 0 ? (uint32_t)d2 : p - (uint32_t)(-d2), m);
+            q0F[l] = td_fix(pf_mont_mul(eM, d0 >= 0 ? (uint32_t)d0 : p - (uint32_t)(-d0), m), fp);
+            q1F[l] = td_fix(pf_mont_mul(eM, d1 >= 0 ? (uint32_t)d1 : p - (uint32_t)(-d1), m), fp);
+            q2F[l] = td_fix(pf_mont_mul(eM, d2 >= 0 ? (uint32_t)d2 : p - (uint32_t)(-d2), m), fp);
         }
 
         const uint32_t qlo = klo / 10u, qhi = khi / 10u;
@@ -523,42 +661,41 @@ void td_large_kernel(const uint32_t *__restrict__ primes, uint32_t n,
             const uint32_t kb = q * 10u;
             const uint32_t dlo = kb < klo ? klo - kb : 0u;
             const uint32_t dhi = min(9u, khi - kb);
-            uint32_t v[NR];
+            uint32_t vF[NR];
 #pragma unroll
-            for (int l = 0; l < NR; l++) v[l] = w[l];
+            for (int l = 0; l < NR; l++) vF[l] = wF[l];
             if (dlo == 0u && dhi == 9u) {
 #pragma unroll
                 for (uint32_t d = 0; d < 10u; d++) {
-                    un = pf_submod(un, bstep, p);
-                    bool h = un < thrn;
+                    uF -= bF;
+                    bool h = uF < tU;
 #pragma unroll
-                    for (int l = 0; l < NR; l++) h |= v[l] < thr[l];
-                    if (__builtin_expect(h, 0))
-                        td_hits<DJ, MERGED, NR>(kb + d, un, thrn, v, thr, J, rdec, bits);
+                    for (int l = 0; l < NR; l++) h |= vF[l] < tV[l];
+                    if (__builtin_expect(td_any(h), 0))
+                        if (h) td_hits_exact<DJ, MERGED, NR>(kb + d, lk, un0, bstepM, cm, eMl, thr, m, J, rdec, bits);
 #pragma unroll
-                    for (int l = 0; l < NR; l++) v[l] = pf_submod(v[l], s0[l], p);
+                    for (int l = 0; l < NR; l++) vF[l] -= sF[l];
                 }
             } else {
                 for (uint32_t d = 0; d < dlo; d++) {
 #pragma unroll
-                    for (int l = 0; l < NR; l++) v[l] = pf_submod(v[l], s0[l], p);
+                    for (int l = 0; l < NR; l++) vF[l] -= sF[l];
                 }
                 for (uint32_t d = dlo; d <= dhi; d++) {
-                    un = pf_submod(un, bstep, p);
-                    bool h = un < thrn;
+                    uF -= bF;
+                    bool h = uF < tU;
 #pragma unroll
-                    for (int l = 0; l < NR; l++) h |= v[l] < thr[l];
-                    if (h) td_hits<DJ, MERGED, NR>(kb + d, un, thrn, v, thr, J, rdec, bits);
+                    for (int l = 0; l < NR; l++) h |= vF[l] < tV[l];
+                    if (h) td_hits_exact<DJ, MERGED, NR>(kb + d, lk, un0, bstepM, cm, eMl, thr, m, J, rdec, bits);
 #pragma unroll
-                    for (int l = 0; l < NR; l++) v[l] = pf_submod(v[l], s0[l], p);
+                    for (int l = 0; l < NR; l++) vF[l] -= sF[l];
                 }
             }
             if (q < qhi) {
                 uint32_t t9 = 0, qq = q;
                 while (qq % 10u == 9u) { t9++; qq /= 10u; }
 #pragma unroll
-                for (int l = 0; l < NR; l++)
-                    w[l] = pf_submod(w[l], t9 == 0 ? tq0[l] : t9 == 1 ? tq1[l] : tq2[l], p);
+                for (int l = 0; l < NR; l++) wF[l] -= t9 == 0 ? q0F[l] : t9 == 1 ? q1F[l] : q2F[l];
             }
         }
         kresM = pf_mont_mul(kresM, tenInv, m);
