@@ -6,7 +6,8 @@ import sys
 
 import pytest
 
-from _support import gmp_build, gmp_probab_prime, gmp_sprp, gmp_version, sprp_python
+from _support import (gmp_build, gmp_powm, gmp_probab_prime, gmp_sprp, gmp_version, inv_pow2_mod,
+                      sprp_python)
 
 MERSENNE_EXP = [13, 17, 19, 31, 61, 89, 107, 127, 521, 607, 1279, 2203, 2281]
 
@@ -78,3 +79,29 @@ def test_known_primes_and_strong_pseudoprimes():
             n = 10000 - 100 * k - 10 * j - 1
             td = all(n % d for d in range(2, int(n ** 0.5) + 1))
             assert (gmp_probab_prime(n) >= 1) == td
+
+
+def test_powm_matches_python():
+    """mr_oracle_powm (GMP mpz_powm with a full exponent; the checker of the
+    checkpoint residues and the Miller-Rabin CPU baseline) against Python's
+    pow, including exponents longer and shorter than the modulus, base 0
+    and base >= N."""
+    rng = random.Random(41)
+    for bits in (17, 64, 300, 2000):
+        for _ in range(6):
+            n = rng.getrandbits(bits) | 1 | (1 << (bits - 1))
+            b = rng.randrange(3 * n)
+            e = rng.getrandbits(rng.choice([1, bits // 2, bits, 2 * bits]))
+            assert gmp_powm(b, e, n) == pow(b, e, n)
+        assert gmp_powm(0, 5, n) == 0
+        assert gmp_powm(7, 0, n) == 1 % n
+
+
+def test_inverse_of_the_montgomery_radix():
+    """inv_pow2_mod (R^-1 mod n by Newton, used to check Montgomery
+    products) against pow(R, -1, n)."""
+    rng = random.Random(42)
+    for bits in (24, 100, 5000):
+        n = rng.getrandbits(bits) | 1 | (1 << (bits - 1))
+        R = 1 << (24 * (bits // 24 + 1))
+        assert inv_pow2_mod(n, R) == pow(R, -1, n)
