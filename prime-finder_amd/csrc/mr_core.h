@@ -587,31 +587,41 @@ struct CrtConst {
 
 // Limbs from the two residue vectors of a product: thread t reads positions
 // [t*C, t*C+C) of Y0 and Y1 (LDS or registers via the functor) and writes
-// normalised limbs to v[]; positions >= keep are zeroed (keep = M for all).
+// normalised limbs to v[]; positions >= keep are zeroed (keep = M for all);
+// inputs at positions >= live are known to be zero.
 template <int C, class Get0, class Get1>
 __device__ __forceinline__ void crt_carry(uint32_t (&v)[C], Get0 get0, Get1 get1,
-                                          const CrtConst &k, int keep, Scratch &sc)
+                                          const CrtConst &k, int keep, Scratch &sc,
+                                          int live = 1 << 30)
 {
     const int t = tid_();
     uint64_t acc = 0;
     // inputs in batches of CB, all of a batch's reads issued before use
-    // (some callers read them from global memory)
+    // (some callers read them from global memory).  A thread whose run lies
+    // wholly at or above keep (its limbs are zeroed below, and a carry out
+    // of it lands above keep as well) or at or above live (all its inputs
+    // are zero) reads and converts nothing; carries still pass through it.
     constexpr int CB = C < 8 ? C : 8;
+    if (t * C >= min(keep, live)) {
 #pragma unroll
-    for (int i0 = 0; i0 < C; i0 += CB) {
-        uint32_t y0[CB], y1[CB];
+        for (int i = 0; i < C; i++) v[i] = 0u;
+    } else {
 #pragma unroll
-        for (int i = 0; i < CB; i++) { y0[i] = get0(i0 + i); y1[i] = get1(i0 + i); }
+        for (int i0 = 0; i0 < C; i0 += CB) {
+            uint32_t y0[CB], y1[CB];
 #pragma unroll
-        for (int i = 0; i < CB; i++) {
-            const uint32_t a0 = shoup(y0[i], k.s0, k.s0q, P0);
-            const uint32_t u = shoup(y1[i], k.c1, k.c1q, P1);
-            const uint32_t w = shoup(a0, k.c2, k.c2q, P1);
-            const uint32_t tt = sub_mod(u, w, P1);
-            const uint64_t x = (uint64_t)tt * P0 + a0;
-            acc += x;
-            v[i0 + i] = (uint32_t)acc & LIMB_MASK;
-            acc >>= LIMB_BITS;
+            for (int i = 0; i < CB; i++) { y0[i] = get0(i0 + i); y1[i] = get1(i0 + i); }
+#pragma unroll
+            for (int i = 0; i < CB; i++) {
+                const uint32_t a0 = shoup(y0[i], k.s0, k.s0q, P0);
+                const uint32_t u = shoup(y1[i], k.c1, k.c1q, P1);
+                const uint32_t w = shoup(a0, k.c2, k.c2q, P1);
+                const uint32_t tt = sub_mod(u, w, P1);
+                const uint64_t x = (uint64_t)tt * P0 + a0;
+                acc += x;
+                v[i0 + i] = (uint32_t)acc & LIMB_MASK;
+                acc >>= LIMB_BITS;
+            }
         }
     }
     sc.carry[t] = acc;               // < 2^40

@@ -308,7 +308,7 @@ __device__ __forceinline__ void tproduct(Smem<LOGM> &sm, const Twiddles &T, cons
                          const uint32_t y = Y1[px(b0 + i)];
                          return in ? y : 0u;
                      },
-                     half ? K.mulh : k, keep, sm.sc);
+                     half ? K.mulh : k, keep, sm.sc, half ? M / 2 : M);
     } else {
         constexpr int H = G<LOGM>::H, HALF_T = THREADS / 2;   // threads covering [0, H)
         static_assert(C * HALF_T == H, "split layout");
@@ -319,7 +319,7 @@ __device__ __forceinline__ void tproduct(Smem<LOGM> &sm, const Twiddles &T, cons
         if (wrap) {
             const bool lo = p0 < H;
             crt_carry<C>(out, [&](int i) { return lo ? X0[px(p0 + i)] : 0u; },
-                         [&](int i) { return lo ? X1[px(p0 + i)] : 0u; }, K.mulh, keep, sm.sc);
+                         [&](int i) { return lo ? X1[px(p0 + i)] : 0u; }, K.mulh, keep, sm.sc, H);
             return;
         }
         {   // u -> ysave: element i of position block t' (n = t' C + i) at
