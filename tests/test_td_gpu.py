@@ -127,6 +127,22 @@ def test_max_grid_sampled_primes_vs_oracle(gpu):
         assert (got == ref).all(), (a, b, c)
 
 
+def test_edge_exponents_large_primes_vs_oracle(gpu):
+    """Exponent-edge triples (separate j digit groups when a - c < D_j, rev
+    terms with negative exponents) over primes from 2^24 up, which take the
+    lane kernel's fixed-point residue path, and next to 2^31."""
+    allp = gpu.primes_below(10 ** 8)
+    primes = np.concatenate([allp[allp >= (1 << 24)][::9973],
+                             np.array([16777259, 2147483647, 2147483629], np.uint32)])
+    for (a, b, c) in [(1003, 1001, 1000), (1002, 1001, 1000), (5000, 4999, 4998),
+                      (1005, 1001, 1000), (50004, 50001, 50000)]:
+        say("edge large", a, b, c, len(primes))
+        for (K, J) in [(1500, 1500), (4096, 4096)]:
+            rc, got = gpu.trial_div_gpu_kj(primes, a, b, c, K, J)
+            assert rc == 0
+            assert (got == oracle_sieve(primes, a, b, c, K, J)).all(), (a, b, c, K, J)
+
+
 def test_workload2_full_triple_properties(gpu):
     """One full workload-2 triple (5 761 455 primes, 4096 x 4096): sampled
     bits against full-list oracle verdicts, determinism, and the OR-split
