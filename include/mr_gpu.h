@@ -69,6 +69,26 @@ mr_ctx_t *mr_ctx_create_sparse(const mr_candidate_t *cands, int n_batch, int dev
 int mr_run(mr_ctx_t *ctx, const uint32_t *bases, int n_bases,
            uint8_t *alive_out, uint32_t *residues_out);
 
+/* mr_run with options.  flags = 0 behaves as mr_run; MR_RUN_ALL_BASES runs
+ * every base on every candidate (no retirement), as the reference runner
+ * computes every witness (miller_rabin_runner.cu:305-354), so residues_out
+ * holds a^d mod N for every (candidate, base); alive_out is then the AND of
+ * the rounds.  pass_out (NULL or [n_batch][n_bases]): 1 where that base's
+ * round passed (without MR_RUN_ALL_BASES: where the candidate was still
+ * alive after it).  Returns 0, -1 (invalid argument) or -2 (HIP error). */
+#define MR_RUN_ALL_BASES 1
+int mr_run_ex(mr_ctx_t *ctx, const uint32_t *bases, int n_bases, int flags,
+              uint8_t *alive_out, uint8_t *pass_out, uint32_t *residues_out);
+
+/* One strong-probable-prime round in which candidate i uses its own base
+ * bases[i] (n_batch entries, each in [2, 2^24)): every (number, base) test
+ * of a batch in one launch, the way the reference runner computes all
+ * witnesses of all candidates (miller_rabin_runner.cu:305-354), with the
+ * numbers repeated in the context once per base.  pass_out[n_batch]: 1 =
+ * the round passed.  residues_out (NULL or [n_batch][n_words]): a^d mod N. */
+int mr_run_per_slot(mr_ctx_t *ctx, const uint32_t *bases, uint8_t *pass_out,
+                    uint32_t *residues_out);
+
 /* Per-context geometry: words per number, limbs L (24-bit), log2 of the
  * transform length, and s (N - 1 = 2^s d) per candidate (may be NULL). */
 int mr_ctx_info(const mr_ctx_t *ctx, int *n_words, int *limbs, int *log_len, int *s_out);

@@ -121,21 +121,6 @@ long mr_oracle_build(long long a, long long b, long long c,
     return r;
 }
 
-/* Decimal digit count of a limb array (exact). */
-long mr_oracle_decimal_digits(const uint32_t *v, size_t n)
-{
-    mpz_t x;
-    mpz_init(x);
-    mo_from_limbs(x, v, n);
-    char *s = mpz_get_str(NULL, 10, x);
-    long L = (long)strlen(s);
-    void (*freefn)(void *, size_t);
-    mp_get_memory_functions(NULL, NULL, &freefn);
-    freefn(s, (size_t)L + 1);
-    mpz_clear(x);
-    return L;
-}
-
 /* out = base^E mod N (all limb arrays of length n, E of length ne). */
 int mr_oracle_powm(const uint32_t *N, size_t n, const uint32_t *base,
                    const uint32_t *E, size_t ne, uint32_t *out)
@@ -155,24 +140,9 @@ int mr_oracle_powm(const uint32_t *N, size_t n, const uint32_t *base,
     return rc;
 }
 
-/* out = A * B mod N. */
-int mr_oracle_mulmod(const uint32_t *A, const uint32_t *B, const uint32_t *N,
-                     size_t n, uint32_t *out)
-{
-    mpz_t x, y, m;
-    mpz_inits(x, y, m, NULL);
-    mo_from_limbs(x, A, n);
-    mo_from_limbs(y, B, n);
-    mo_from_limbs(m, N, n);
-    mpz_mul(x, x, y);
-    mpz_mod(x, x, m);
-    int rc = mo_to_limbs(x, out, n) < 0 ? -1 : 0;
-    mpz_clears(x, y, m, NULL);
-    return rc;
-}
-
 /* Strong probable-prime round to `base`.  Writes base^d mod N into resid
- * (n limbs, may be NULL) and s into *s_out (may be NULL).
+ * (n limbs, may be NULL) and s into *s_out (may be NULL).  A base that is a
+ * multiple of N counts as passed (it says nothing about N).
  * Returns 1 = pass, 0 = composite witness, -1 = bad input (N < 5 or even). */
 int mr_oracle_sprp(const uint32_t *N, size_t n, unsigned long base,
                    uint32_t *resid, int *s_out)
@@ -190,6 +160,8 @@ int mr_oracle_sprp(const uint32_t *N, size_t n, unsigned long base,
     mpz_mod(bb, bb, m);
     mpz_powm(r, bb, d, m);
     if (resid && mo_to_limbs(r, resid, n) < 0) { rc = -1; goto out; }
+    /* a base that is a multiple of N tests nothing: the round passes */
+    if (mpz_sgn(bb) == 0) { rc = 1; goto out; }
     if (mpz_cmp_ui(r, 1u) == 0 || mpz_cmp(r, m1) == 0) { rc = 1; goto out; }
     rc = 0;
     for (unsigned long i = 1; i < s; i++) {

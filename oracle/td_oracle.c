@@ -195,24 +195,6 @@ int td_oracle_brute(const uint32_t *primes, uint32_t n_primes,
     return 0;
 }
 
-/* Residues used by the survivor spot checks: N mod p and formula rev(N) mod
- * p for one (k, j), with -1 meaning "rev term gated off". */
-void td_oracle_residues(uint32_t p, long long a, long long b, long long c,
-                        uint32_t k, uint32_t j, long long *n_mod, long long *rev_mod)
-{
-    const uint64_t P = p;
-    const uint64_t ra = or_powmod(10u, (uint64_t)a, P);
-    const uint64_t rb = or_powmod(10u, (uint64_t)b, P);
-    const uint64_t rc = or_powmod(10u, (uint64_t)c, P);
-    *n_mod = (long long)((ra + 3u * P - (uint64_t)k * rb % P - (uint64_t)j * rc % P - 1u) % P);
-    const int lk = or_ndigits(k), lj = or_ndigits(j);
-    const uint64_t kr = or_rev_residue(a - b - lk, p);
-    const uint64_t jr = or_rev_residue(a - c - lj, p);
-    if (kr == 0 || jr == 0) { *rev_mod = -1; return; }
-    const uint64_t rk = or_reverse(k, lk), rj = or_reverse(j, lj);
-    *rev_mod = (long long)((ra + 3u * P - rk * kr % P - rj * jr % P - 1u) % P);
-}
-
 /* Full-list verdicts for a sample of (k, j) pairs: flags[i] = 1 iff some
  * prime of the list eliminates (ks[i], js[i]) under the bitmap contract above
  * (direct residue evaluation, as td_oracle_brute).  Threads split the prime

@@ -836,6 +836,7 @@ __global__ __launch_bounds__(THREADS) void setup_kernel(Dev D, Twiddles T, Const
 //   base of `bases`.
 // mode 2 (scheduler, after round_end): next base of `bases` if the slot is
 //   alive and one is left, else the verdict is final (done = 1).
+// mode 3: every alive slot, with its own base bases[slot].
 template <int LOGM>
 __global__ __launch_bounds__(THREADS) void round_start_kernel(Dev D, uint32_t base, int mode,
                                                               const uint32_t *bases, int n_bases)
@@ -845,7 +846,11 @@ __global__ __launch_bounds__(THREADS) void round_start_kernel(Dev D, uint32_t ba
     __shared__ uint32_t stash[Lds<LOGM>::MP];
     __shared__ uint32_t go;
     const int c = slot_id(D);
-    if (mode != 0) {
+    if (mode == 3) {
+        if (!D.alive[c] || D.pre[c]) return;
+        base = bases[c];
+        if (threadIdx.x == 0) D.base[c] = base;
+    } else if (mode != 0) {
         if (threadIdx.x == 0) {
             int bi = 0;
             bool run;
@@ -925,8 +930,12 @@ __device__ __forceinline__ bool vec_equal(const uint32_t (&a)[C], const uint32_t
     return !block_any(ne, sc);
 }
 
-// End of a round: pass iff r == R or r^(2^i) == (N-1) R for some i < s;
-// optionally writes r out of Montgomery form (a^d mod N) to out.
+// End of a round: pass iff r == R or r^(2^i) == (N-1) R for some i < s, or
+// r == 0, which happens only when the base is a multiple of N (N < 2^24,
+// e.g. N = 5 or 7 with base 5 or 7): such a base says nothing about N and
+// the round counts as passed, as mpz_probab_prime_p reports 5 and 7 prime
+// (miller_rabin.c:91).  Optionally writes r out of Montgomery form (a^d mod
+// N) to out.
 template <int LOGM>
 __global__ __launch_bounds__(THREADS) void round_end_kernel(Dev D, Twiddles T, Consts K,
                                                             uint32_t *out, size_t out_stride)
@@ -940,7 +949,8 @@ __global__ __launch_bounds__(THREADS) void round_end_kernel(Dev D, Twiddles T, C
     const uint32_t *nm1 = D.nm1 + (size_t)c * D.stride;
     uint32_t r[C];
     load_vec<C>(r, D.res + (size_t)c * D.stride, L, L);
-    bool pass = vec_equal_mem<C>(r, one, L, L, sm.sc) || vec_equal_mem<C>(r, nm1, L, L, sm.sc);
+    bool pass = vec_equal_mem<C>(r, one, L, L, sm.sc) || vec_equal_mem<C>(r, nm1, L, L, sm.sc) ||
+                vec_equal_mem<C>(r, one, 0, L, sm.sc);          // r == 0: base == 0 (mod N)
     if (out) {   // a^d mod N = REDC(r): product with 1
         uint32_t *one_f = D.scr + (size_t)c * 2 * G<LOGM>::M;
         {
@@ -1362,23 +1372,44 @@ extern "C" int mr_ctx_moduli(const mr_ctx_t *x, uint32_t *out)
     return e == hipSuccess ? 0 : hip_fail(e, "mr_ctx_moduli");
 }
 
-extern "C" int mr_run(mr_ctx_t *x, const uint32_t *bases, int n_bases, uint8_t *alive_out,
-                      uint32_t *residues_out)
+namespace {
+
+// Rounds of every base in order over the whole context.  all_bases = false:
+// a candidate retires at its first failing base (its later rounds are not
+// run); true: every base is run on every candidate, as the reference runner
+// does for each witness (miller_rabin_runner.cu:305-354), and alive_out is
+// the AND over the bases.  pass_out (nullable, [n][n_bases]): 1 where the
+// round of that base passed (retiring: where the candidate was still alive
+// after it).
+// per_slot: one round in which slot c uses base bases[c] (n_bases == n).
+int run_rounds(mr_ctx *x, const uint32_t *bases, int n_bases, bool all_bases,
+               uint8_t *alive_out, uint8_t *pass_out, uint32_t *residues_out,
+               bool per_slot = false)
 {
     if (!x || !bases || n_bases < 0 || !alive_out) return -1;
+    if (per_slot && n_bases != x->n) return -1;
     for (int i = 0; i < n_bases; i++)
         if (bases[i] < 2u || bases[i] >= (1u << 24)) return -1;
     MR_CHECK(hipSetDevice(x->device));
     const int n = x->n;
+    uint32_t *dbases = nullptr;
+    if (per_slot) {
+        MR_CHECK(hipMalloc(&dbases, sizeof(uint32_t) * (size_t)n));
+        MR_CHECK(hipMemcpyAsync(dbases, bases, sizeof(uint32_t) * (size_t)n,
+                                hipMemcpyHostToDevice, x->st));
+        n_bases = 1;     // one round
+    }
     init_alive_kernel<<<(n + 255) / 256, 256, 0, x->st>>>(x->d);
     uint32_t *dres = nullptr;
     uint32_t *dlimbs = nullptr;
+    uint8_t *dpass = nullptr;
     const size_t rbytes = sizeof(uint32_t) * (size_t)n * n_bases * x->n_words;
     if (residues_out) {
         MR_CHECK(hipMalloc(&dres, rbytes ? rbytes : 4));
         MR_CHECK(hipMemsetAsync(dres, 0, rbytes ? rbytes : 4, x->st));
         MR_CHECK(hipMalloc(&dlimbs, sizeof(uint32_t) * (size_t)n * x->stride));
     }
+    if (pass_out || all_bases) MR_CHECK(hipMalloc(&dpass, (size_t)n * (n_bases ? n_bases : 1)));
     int max_bits = 0;
     {
         std::vector<int32_t> nb(n);
@@ -1390,9 +1421,13 @@ extern "C" int mr_run(mr_ctx_t *x, const uint32_t *bases, int n_bases, uint8_t *
     int rc = 0;
     for (int bi = 0; bi < n_bases && rc == 0; bi++) {
         const uint32_t base = bases[bi];
+        if (all_bases && bi > 0) init_alive_kernel<<<(n + 255) / 256, 256, 0, x->st>>>(x->d);
         rc = dispatch_logm(x->logm, [&](auto lg) {
             constexpr int LG = decltype(lg)::value;
-            round_start_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, base, 0, nullptr, 0);
+            if (per_slot)
+                round_start_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, 0u, 3, dbases, n);
+            else
+                round_start_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, base, 0, nullptr, 0);
             for (int done = 0; done < max_bits; done += CHUNK)
                 exp_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, x->T, x->K, CHUNK);
             if (residues_out) {
@@ -1404,6 +1439,9 @@ extern "C" int mr_run(mr_ctx_t *x, const uint32_t *bases, int n_bases, uint8_t *
             } else {
                 round_end_kernel<LG><<<n, THREADS, 0, x->st>>>(x->d, x->T, x->K, nullptr, 0);
             }
+            if (dpass)
+                MR_CHECK(hipMemcpyAsync(dpass + (size_t)bi * n, x->d.alive, n,
+                                        hipMemcpyDeviceToDevice, x->st));
             return 0;
         });
         if (rc == 0) {
@@ -1411,16 +1449,57 @@ extern "C" int mr_run(mr_ctx_t *x, const uint32_t *bases, int n_bases, uint8_t *
             if (e != hipSuccess) rc = hip_fail(e, "mr_run launch");
         }
     }
+    std::vector<uint8_t> hpass;
     if (rc == 0) {
         hipError_t e = hipMemcpyAsync(alive_out, x->d.alive, n, hipMemcpyDeviceToHost, x->st);
         if (e == hipSuccess && residues_out)
             e = hipMemcpyAsync(residues_out, dres, rbytes, hipMemcpyDeviceToHost, x->st);
+        if (e == hipSuccess && dpass && n_bases > 0) {
+            hpass.resize((size_t)n * n_bases);
+            e = hipMemcpyAsync(hpass.data(), dpass, hpass.size(), hipMemcpyDeviceToHost, x->st);
+        }
         if (e == hipSuccess) e = hipStreamSynchronize(x->st);
         if (e != hipSuccess) rc = hip_fail(e, "mr_run");
     }
+    if (rc == 0 && !hpass.empty()) {
+        for (int c = 0; c < n; c++) {
+            bool all = true;
+            for (int bi = 0; bi < n_bases; bi++) {
+                const uint8_t v = hpass[(size_t)bi * n + c];
+                all = all && v;
+                if (pass_out) pass_out[(size_t)c * n_bases + bi] = v ? 1 : 0;
+            }
+            if (all_bases) alive_out[c] = all ? 1 : 0;
+        }
+    }
     if (dres) (void)hipFree(dres);
     if (dlimbs) (void)hipFree(dlimbs);
+    if (dpass) (void)hipFree(dpass);
+    if (dbases) (void)hipFree(dbases);
     return rc;
+}
+
+} // namespace
+
+extern "C" int mr_run(mr_ctx_t *x, const uint32_t *bases, int n_bases, uint8_t *alive_out,
+                      uint32_t *residues_out)
+{
+    return run_rounds(x, bases, n_bases, false, alive_out, nullptr, residues_out);
+}
+
+extern "C" int mr_run_per_slot(mr_ctx_t *x, const uint32_t *bases, uint8_t *pass_out,
+                               uint32_t *residues_out)
+{
+    if (!x || !pass_out) return -1;
+    return run_rounds(x, bases, x->n, false, pass_out, nullptr, residues_out, true);
+}
+
+extern "C" int mr_run_ex(mr_ctx_t *x, const uint32_t *bases, int n_bases, int flags,
+                         uint8_t *alive_out, uint8_t *pass_out, uint32_t *residues_out)
+{
+    if (flags & ~MR_RUN_ALL_BASES) return -1;
+    return run_rounds(x, bases, n_bases, (flags & MR_RUN_ALL_BASES) != 0, alive_out, pass_out,
+                      residues_out);
 }
 
 namespace {

@@ -37,6 +37,11 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
+#include <atomic>
+#include <thread>
+#include <vector>
+
 namespace {
 
 constexpr uint32_t TD_SMALL_LIMIT = 1u << 16;   // row kernel below, lane kernel above
@@ -710,6 +715,7 @@ constexpr int TD_MAX_DEV = 64;
 constexpr uint32_t TD_RDEC_OFF[5] = {0u, 0u, 10u, 110u, 1110u};
 
 struct TdScratch {
+    bool ready = false;          // every field below allocated and initialised
     uint32_t *small = nullptr;   // [TD_SMALL_CAP]
     uint32_t *tbl = nullptr;     // [TD_SMALL_CAP * TD_TBL]
     uint32_t *ctrl = nullptr;    // [CT_WORDS]
@@ -721,23 +727,46 @@ struct TdScratch {
 
 TdScratch g_scratch[TD_MAX_DEV];
 
+void td_scratch_release(TdScratch &s)
+{
+    if (s.small) (void)hipFree(s.small);
+    if (s.tbl) (void)hipFree(s.tbl);
+    if (s.ctrl) (void)hipFree(s.ctrl);
+    if (s.rdec) (void)hipFree(s.rdec);
+    if (s.items) (void)hipFree(s.items);
+    if (s.aux) (void)hipStreamDestroy(s.aux);
+    if (s.fork) (void)hipEventDestroy(s.fork);
+    if (s.join) (void)hipEventDestroy(s.join);
+    s = TdScratch{};
+}
+
+// Per-device scratch, set up on first use.  A failed set-up releases what it
+// allocated, so the next call tries again from scratch.
 int td_scratch(TdScratch **out)
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= TD_MAX_DEV) return -1;
     TdScratch &s = g_scratch[dev];
-    if (!s.ctrl) {
-        if (hipMalloc(&s.small, TD_SMALL_CAP * sizeof(uint32_t)) != hipSuccess) return -1;
-        if (hipMalloc(&s.tbl, (size_t)TD_SMALL_CAP * TD_TBL * sizeof(uint32_t)) != hipSuccess) return -1;
-        if (hipMalloc(&s.ctrl, CT_WORDS * sizeof(uint32_t)) != hipSuccess) return -1;
-        if (hipMemset(s.ctrl, 0, CT_WORDS * sizeof(uint32_t)) != hipSuccess) return -1;
-        if (hipMalloc(&s.rdec, TD_RDEC_SIZE * sizeof(uint16_t)) != hipSuccess) return -1;
-        if (hipMalloc(&s.items, TD_ITEM_CAP * sizeof(uint32_t)) != hipSuccess) return -1;
-        td_rdec_kernel<<<(TD_RDEC_SIZE + 255u) / 256u, 256>>>(s.rdec);
-        if (hipStreamCreateWithFlags(&s.aux, hipStreamNonBlocking) != hipSuccess) return -1;
-        if (hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) != hipSuccess) return -1;
-        if (hipEventCreateWithFlags(&s.join, hipEventDisableTiming) != hipSuccess) return -1;
-        if (hipDeviceSynchronize() != hipSuccess) return -1;
+    if (!s.ready) {
+        bool ok = hipMalloc(&s.small, TD_SMALL_CAP * sizeof(uint32_t)) == hipSuccess &&
+                  hipMalloc(&s.tbl, (size_t)TD_SMALL_CAP * TD_TBL * sizeof(uint32_t)) == hipSuccess &&
+                  hipMalloc(&s.ctrl, CT_WORDS * sizeof(uint32_t)) == hipSuccess &&
+                  hipMemset(s.ctrl, 0, CT_WORDS * sizeof(uint32_t)) == hipSuccess &&
+                  hipMalloc(&s.rdec, TD_RDEC_SIZE * sizeof(uint16_t)) == hipSuccess &&
+                  hipMalloc(&s.items, TD_ITEM_CAP * sizeof(uint32_t)) == hipSuccess;
+        if (ok) {
+            td_rdec_kernel<<<(TD_RDEC_SIZE + 255u) / 256u, 256>>>(s.rdec);
+            ok = hipGetLastError() == hipSuccess &&
+                 hipStreamCreateWithFlags(&s.aux, hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) == hipSuccess &&
+                 hipEventCreateWithFlags(&s.join, hipEventDisableTiming) == hipSuccess &&
+                 hipDeviceSynchronize() == hipSuccess;
+        }
+        if (!ok) {
+            td_scratch_release(s);
+            return -1;
+        }
+        s.ready = true;
     }
     *out = &s;
     return 0;
@@ -800,12 +829,14 @@ int td_launch(const uint32_t *d_primes, uint32_t n, long long a, long long b, lo
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// Device copy of a host prime list, reused while (pointer, count, checksum)
-// are unchanged.
+// Device copy of a host prime list, reused while the caller's list is
+// unchanged: same pointer, same count and the same contents, compared word
+// for word against a host copy kept from the last upload (no hash, so no
+// collision can reuse a stale list).
 struct TdPrimeCache {
     const uint32_t *host = nullptr;
     uint32_t count = 0;
-    uint64_t sum = 0;
+    uint32_t *copy = nullptr;    // host copy of the uploaded list
     uint32_t *dev = nullptr;
     size_t cap = 0;
     uint32_t *bits = nullptr;
@@ -814,14 +845,31 @@ struct TdPrimeCache {
 
 TdPrimeCache g_cache[TD_MAX_DEV];
 
-uint64_t td_checksum(const uint32_t *v, uint32_t n)
+// a[0..n) == b[0..n).  The full list below 10^8 is 23 MB, which one core
+// reads in about 3 ms, so lists above 4 MB are split over up to 8 threads.
+bool td_same_words(const uint32_t *a, const uint32_t *b, uint32_t n)
 {
-    uint64_t h0 = 0x9E3779B97F4A7C15ull, h1 = n;
-    for (uint32_t i = 0; i < n; i++) {
-        h0 = (h0 ^ v[i]) * 0x100000001B3ull;
-        h1 += (uint64_t)v[i] * (2u * i + 1u);
+    const size_t bytes = (size_t)n * sizeof(uint32_t);
+    if (bytes < ((size_t)4 << 20)) return memcmp(a, b, bytes) == 0;
+    unsigned nt = std::thread::hardware_concurrency();
+    nt = nt < 2u ? 2u : nt > 8u ? 8u : nt;
+    const size_t per = (n + nt - 1u) / nt;
+    std::atomic<bool> differ{false};
+    auto part = [&](unsigned t) {
+        const size_t lo = (size_t)t * per, hi = std::min((size_t)n, lo + per);
+        if (lo < hi && memcmp(a + lo, b + lo, (hi - lo) * sizeof(uint32_t)) != 0)
+            differ.store(true, std::memory_order_relaxed);
+    };
+    std::vector<std::thread> th;
+    try {
+        for (unsigned t = 1; t < nt; t++) th.emplace_back(part, t);
+    } catch (...) {
+        for (auto &x : th) x.join();
+        return memcmp(a, b, bytes) == 0;
     }
-    return h0 ^ (h1 << 1);
+    part(0);
+    for (auto &x : th) x.join();
+    return !differ.load();
 }
 
 double td_secs(hipEvent_t x, hipEvent_t y)
@@ -836,7 +884,7 @@ int td_check_error_code()
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= TD_MAX_DEV) return -1;
     TdScratch &s = g_scratch[dev];
-    if (!s.ctrl) return 0;
+    if (!s.ready) return 0;
     uint32_t err = 0;
     if (hipMemcpy(&err, s.ctrl + CT_ERROR, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess) return -1;
     if (err) {
@@ -872,15 +920,22 @@ int td_run_host(const prime_list_t *pl, long long a, long long b, long long c,
     PF_TRY(who, hipEventCreate(&e2));
     PF_TRY(who, hipEventCreate(&e3));
     {
-        const uint64_t sum = td_checksum(pl->primes, pl->count);
-        fresh = !(pc->dev && pc->host == pl->primes && pc->count == pl->count && pc->sum == sum);
+        fresh = !(pc->dev && pc->host == pl->primes && pc->count == pl->count &&
+                  (pl->count == 0 || td_same_words(pc->copy, pl->primes, pl->count)));
         if (fresh) {
             pc->host = nullptr;  // invalid until the copy below succeeds
             if (pc->cap < pl->count) {
                 if (pc->dev) PF_TRY(who, hipFree(pc->dev));
+                free(pc->copy);
                 pc->dev = nullptr;
+                pc->copy = nullptr;
                 pc->cap = 0;
                 PF_TRY(who, hipMalloc(&pc->dev, (size_t)pl->count * sizeof(uint32_t) + 4));
+                pc->copy = (uint32_t *)malloc((size_t)pl->count * sizeof(uint32_t) + 4);
+                if (!pc->copy) {
+                    fprintf(stderr, "%s: out of host memory\n", who);
+                    goto fail;
+                }
                 pc->cap = pl->count;
             }
             if (!pc->dev) PF_TRY(who, hipMalloc(&pc->dev, 4));
@@ -898,9 +953,9 @@ int td_run_host(const prime_list_t *pl, long long a, long long b, long long c,
                                   hipMemcpyHostToDevice));
         PF_TRY(who, hipMemcpy(pc->bits, bits, bb, hipMemcpyHostToDevice));
         if (fresh) {
+            if (pl->count) memcpy(pc->copy, pl->primes, (size_t)pl->count * sizeof(uint32_t));
             pc->host = pl->primes;
             pc->count = pl->count;
-            pc->sum = sum;
         }
         PF_TRY(who, hipEventRecord(e1, nullptr));
         if (td_launch(pc->dev, pl->count, a, b, c, K, J, pc->bits, nullptr) != 0) {

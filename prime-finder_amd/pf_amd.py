@@ -220,6 +220,8 @@ def _mr_sigs(L):
         "mr_ctx_create_limbs16": (vp, [C.POINTER(C.c_uint64), i32, i32, i32]),
         "mr_ctx_create_sparse": (vp, [C.POINTER(MrCandidate), i32, i32]),
         "mr_run": (i32, [vp, u32p, i32, C.POINTER(C.c_uint8), u32p]),
+        "mr_run_per_slot": (i32, [vp, u32p, C.POINTER(C.c_uint8), u32p]),
+        "mr_run_ex": (i32, [vp, u32p, i32, i32, C.POINTER(C.c_uint8), C.POINTER(C.c_uint8), u32p]),
         "mr_ctx_info": (i32, [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
                               C.POINTER(C.c_int32)]),
         "mr_ctx_moduli": (i32, [vp, u32p]),
@@ -428,21 +430,53 @@ class MrContext:
         assert self.L.mr_ctx_moduli(self.ptr, out.ctypes.data_as(u32p)) == 0
         return [words_to_int(out[i * self.n_words:(i + 1) * self.n_words]) for i in range(self.n)]
 
-    def run(self, bases, residues: bool = True):
+    def run(self, bases, residues: bool = True, all_bases: bool = False, passes: bool = False):
+        """Rounds of each base (mr_run; all_bases: mr_run_ex with
+        MR_RUN_ALL_BASES, every base on every candidate).  Returns (alive,
+        residues or None) and, with passes, the per-base pass flags too."""
         bases = np.asarray(bases, dtype=np.uint32)
         alive = np.zeros(self.n, np.uint8)
         res = np.zeros(self.n * len(bases) * self.n_words, np.uint32) if residues else None
-        rc = self.L.mr_run(self.ptr, bases.ctypes.data_as(u32p), len(bases),
-                           alive.ctypes.data_as(C.POINTER(C.c_uint8)),
-                           res.ctypes.data_as(u32p) if residues else None)
+        pas = np.zeros(self.n * max(1, len(bases)), np.uint8)
+        u8 = C.POINTER(C.c_uint8)
+        if all_bases or passes:
+            rc = self.L.mr_run_ex(self.ptr, bases.ctypes.data_as(u32p), len(bases),
+                                  1 if all_bases else 0, alive.ctypes.data_as(u8),
+                                  pas.ctypes.data_as(u8),
+                                  res.ctypes.data_as(u32p) if residues else None)
+        else:
+            rc = self.L.mr_run(self.ptr, bases.ctypes.data_as(u32p), len(bases),
+                               alive.ctypes.data_as(u8), res.ctypes.data_as(u32p) if residues else None)
         if rc != 0:
             raise RuntimeError(f"mr_run failed ({rc})")
-        if not residues:
-            return alive.astype(bool).tolist(), None
-        nw, nb = self.n_words, len(bases)
-        rr = [[words_to_int(res[(i * nb + b) * nw:(i * nb + b + 1) * nw]) for b in range(nb)]
-              for i in range(self.n)]
+        rr = None
+        if residues:
+            nw, nb = self.n_words, len(bases)
+            rr = [[words_to_int(res[(i * nb + b) * nw:(i * nb + b + 1) * nw]) for b in range(nb)]
+                  for i in range(self.n)]
+        if passes:
+            nb = len(bases)
+            pp = [[bool(pas[i * nb + b]) for b in range(nb)] for i in range(self.n)]
+            return alive.astype(bool).tolist(), rr, pp
         return alive.astype(bool).tolist(), rr
+
+    def run_per_slot(self, bases, residues: bool = True):
+        """One round, candidate i with base bases[i] (mr_run_per_slot).
+        Returns (pass flags, residues or None)."""
+        bases = np.asarray(bases, dtype=np.uint32)
+        assert len(bases) == self.n
+        ok = np.zeros(self.n, np.uint8)
+        res = np.zeros(self.n * self.n_words, np.uint32) if residues else None
+        rc = self.L.mr_run_per_slot(self.ptr, bases.ctypes.data_as(u32p),
+                                    ok.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                    res.ctypes.data_as(u32p) if residues else None)
+        if rc != 0:
+            raise RuntimeError(f"mr_run_per_slot failed ({rc})")
+        rr = None
+        if residues:
+            nw = self.n_words
+            rr = [words_to_int(res[i * nw:(i + 1) * nw]) for i in range(self.n)]
+        return ok.astype(bool).tolist(), rr
 
     def _words(self, xs):
         return np.concatenate([int_to_words(int(x), self.n_words) for x in xs])

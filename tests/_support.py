@@ -51,11 +51,25 @@ def oracle() -> C.CDLL:
         L.td_oracle_check_pairs.argtypes = [u32p, C.c_uint32, ll, ll, ll, u32p, u32p, C.c_uint32,
                                             u8p, C.c_int]
         L.td_oracle_check_pairs.restype = C.c_int
-        L.td_oracle_residues.argtypes = [C.c_uint32, ll, ll, ll, C.c_uint32, C.c_uint32,
-                                         C.POINTER(ll), C.POINTER(ll)]
-        L.td_oracle_residues.restype = None
         _oracle = L
     return _oracle
+
+
+TEST_GPU_PRIMES = [3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71,
+                   73, 79, 83, 89, 97]
+
+
+def golden_cases():
+    """(case, primes, bitmap words) of tests/golden/td_ref_k100.json: bitmaps
+    of the reference's own trial_div_cpu.c at K = J = 100."""
+    import json
+    data = json.loads((GOLDEN / "td_ref_k100.json").read_text())
+    p20k = numpy_primes(20000)
+    for case in data["cases"]:
+        primes = np.array(TEST_GPU_PRIMES, np.uint32) if case["primes"] == "test_gpu" else p20k
+        words = np.array([int(case["bits"][i:i + 8], 16) for i in range(0, len(case["bits"]), 8)],
+                         np.uint32)
+        yield case, primes, words
 
 
 def nwords(K: int, J: int) -> int:
@@ -102,9 +116,16 @@ class RefPrimeList(C.Structure):
 _ref_td = None
 
 
+def have_ref_td() -> bool:
+    """True when oracle/_ref holds the reference's own trial_div_cpu (built in
+    the container that has the reference sources; never shipped to a GPU
+    run).  Checks the file only: nothing is loaded."""
+    return REF_TD_LIB.exists()
+
+
 def ref_td():
     """The reference's own trial_div_cpu (K = J = 100) built from its sources,
-    or None when oracle/_ref was not built."""
+    loaded on first use, or None when oracle/_ref was not built."""
     global _ref_td
     if _ref_td is None and REF_TD_LIB.exists():
         L = C.CDLL(str(REF_TD_LIB))
@@ -241,6 +262,8 @@ def sprp_python(N: int, base: int):
         d //= 2
         s += 1
     r = pow(base, d, N)
+    if base % N == 0:          # a multiple of N says nothing about N
+        return True, r, s
     if r == 1 or r == N - 1:
         return True, r, s
     x = r

@@ -162,33 +162,6 @@ def test_multiplicative_consistency_20k_digits(gpu):
     ctx.close()
 
 
-def test_50k_digit_residues_vs_gmp_golden(gpu):
-    """Workload-3 candidates at 50 000 digits (N and its reversal, built on
-    the device): base-2 residues equal GMP's (SHA-256 digests committed under
-    tests/golden, made by tests/golden/make_mr_golden.py)."""
-    import hashlib
-    import json
-    from _support import GOLDEN
-    gold = json.loads((GOLDEN / "mr_50k_residues.json").read_text())["rows"]
-    gold = [g for g in gold if g["base"] == 2]
-    cands = [(g["a"], g["b"], g["c"], g["k"], g["j"], g["which"]) for g in gold]
-    ctx = gpu.MrContext(candidates=cands)
-    say("50k-digit golden: L", ctx.limbs, "logM", ctx.log_len, "- one base-2 round (~1 min)")
-    mods = ctx.moduli()
-    alive, res = ctx.run([2])
-    say("50k-digit golden: round done")
-    for g, n, ok, r in zip(gold, mods, alive, res):
-        nw = g["n_words"]
-        assert hashlib.sha256(n.to_bytes(4 * nw, "little")).hexdigest() == g["n_sha256"]
-        if n % 2 == 0:
-            assert not ok
-            continue
-        assert ctx.s[mods.index(n)] == g["s"]
-        assert hashlib.sha256(r[0].to_bytes(4 * nw, "little")).hexdigest() == g["residue_sha256"]
-        assert ok == g["pass"]
-    ctx.close()
-
-
 def test_device_builder_matches_gmp_and_python(gpu):
     """N, its decimal reversal and the reference's rev formula, built on the
     device from (a, b, c, k, j), equal GMP's mpz construction (mpz_ui_pow_ui
@@ -224,6 +197,26 @@ def test_even_and_tiny_moduli_are_decided(gpu):
     alive, res = ctx.run([2, 3])
     assert alive == [True, True, False, False, False, True, True]
     ctx.close()
+
+
+def test_bases_that_are_multiples_of_n(gpu):
+    """N = 5 and 7 with bases 2, 3, 5, 7: the base equal to N tests nothing
+    and the round passes (GMP mpz_probab_prime_p calls both prime); 25 and
+    49 stay composite."""
+    from _support import gmp_probab_prime, sprp_python
+    nums = [5, 7, 11, 13, 25, 49, 35]
+    ctx = gpu.MrContext(nums)
+    alive, res = ctx.run([2, 3, 5, 7])
+    for n, a in zip(nums, alive):
+        want = all(sprp_python(n, b)[0] for b in (2, 3, 5, 7))
+        assert a == want, n
+        if n in (5, 7, 11, 13):
+            assert a and gmp_probab_prime(n) >= 1, n
+    ctx.close()
+    v, _ = gpu.mr_test_batch([(1, 0, 0, 1, 1, 0)], [2, 3, 5, 7])   # N = 10 - 1 - 1 - 1 = 7
+    assert v == [1]
+    code, _ = gpu.miller_rabin_test(1, 0, 0, 1, 1)                # N = rev(N) = 7
+    assert code == 2
 
 
 @pytest.mark.parametrize("logm", [12, 13, 15])

@@ -6,9 +6,8 @@ import random
 import numpy as np
 import pytest
 
-from _support import (GOLDEN, bit_get, nwords, numpy_primes, oracle_check_pairs, oracle_sieve,
+from _support import (GOLDEN, bit_get, golden_cases, nwords, numpy_primes, oracle_check_pairs, oracle_sieve,
                       popcount, survivors)
-from test_td_oracle import golden_cases
 
 pytestmark = pytest.mark.gpu
 
@@ -42,12 +41,29 @@ def test_reference_abi_timing_and_or_semantics(gpu):
     # second call: device prime cache hit; same answer
     rc2, got2 = gpu.trial_div_gpu(primes, 50001, 33334, 2749, bits=pre)
     assert rc2 == 0 and (got2 == got).all()
-    # contents changed at the same address and count: cache must refresh
+    # a new buffer with other contents: cache must refresh
     p2 = primes.copy()
     p2[-1] = 3
     rc3, got3 = gpu.trial_div_gpu(p2, 50001, 33334, 2749)
     assert rc3 == 0
     assert (got3 == oracle_sieve(p2, 50001, 33334, 2749)).all()
+
+
+def test_prime_cache_same_address_refresh(gpu):
+    """The device prime list is reused while the caller's list is unchanged;
+    contents changed in place (same pointer, same count) must be seen, at
+    the first entry, the last and in the middle."""
+    primes = numpy_primes(300000)
+    a, b, c = 50001, 33334, 2749
+    rc, first = gpu.trial_div_gpu(primes, a, b, c)
+    assert rc == 0 and (first == oracle_sieve(primes, a, b, c)).all()
+    addr = primes.ctypes.data
+    for pos, val in ((len(primes) - 1, 3), (0, 7), (len(primes) // 2, 11)):
+        primes[pos] = val
+        rc, got = gpu.trial_div_gpu(primes, a, b, c)
+        assert primes.ctypes.data == addr
+        assert rc == 0
+        assert (got == oracle_sieve(primes, a, b, c)).all(), pos
 
 
 @pytest.mark.parametrize("K,J", [(1, 1), (9, 11), (100, 100), (150, 130), (1200, 37),
@@ -173,6 +189,24 @@ def test_workload2_full_triple_properties(gpu):
     flags = oracle_check_pairs(allp, a, b, c, [p[0] for p in pairs], [p[1] for p in pairs])
     for (k, j), f in zip(pairs, flags):
         assert bit_get(full, J, k, j) == int(f), (k, j)
+
+
+def test_workload2_all_triples_bitexact(gpu):
+    """The full trial-division workload: all 8 seeded triples, K = J = 4096,
+    every prime below 10^8 (5 761 455, resident on the device across
+    calls).  Each GPU bitmap's SHA-256 equals the oracle's full-list bitmap
+    (tests/golden/td_full_4096.json, made by make_td_full_golden.py)."""
+    import hashlib
+    gold = json.loads((GOLDEN / "td_full_4096.json").read_text())
+    allp = gpu.primes_below(10 ** 8)
+    assert len(allp) == 5761455
+    assert [[t["a"], t["b"], t["c"]] for t in gold["triples"]] == CONFIGS["2"]["triples"]
+    for t in gold["triples"]:
+        rc, got = gpu.trial_div_gpu_kj(allp, t["a"], t["b"], t["c"], 4096, 4096)
+        assert rc == 0
+        say("workload2", t["a"], t["b"], t["c"], "survivors", 4096 * 4096 - popcount(got))
+        assert popcount(got) == t["eliminated"]
+        assert hashlib.sha256(got.tobytes()).hexdigest() == t["bits_sha256"], t
 
 
 def test_device_resident_launch(gpu):

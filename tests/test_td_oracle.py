@@ -8,22 +8,9 @@ import random
 import numpy as np
 import pytest
 
-from _support import (GOLDEN, bit_get, numpy_primes, oracle_brute, oracle_check_pairs,
-                      oracle_sieve, popcount, ref_td, ref_trial_div_cpu, survivors)
-
-TEST_GPU_PRIMES = [3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37, 41, 43, 47, 53, 59, 61, 67, 71,
-                   73, 79, 83, 89, 97]
-
-
-def golden_cases():
-    data = json.loads((GOLDEN / "td_ref_k100.json").read_text())
-    p20k = numpy_primes(20000)
-    for case in data["cases"]:
-        primes = np.array(TEST_GPU_PRIMES, np.uint32) if case["primes"] == "test_gpu" else p20k
-        words = np.array([int(case["bits"][i:i + 8], 16) for i in range(0, len(case["bits"]), 8)],
-                         np.uint32)
-        yield case, primes, words
-
+from _support import (GOLDEN, TEST_GPU_PRIMES, bit_get, golden_cases, have_ref_td,
+                      numpy_primes, oracle_brute, oracle_check_pairs, oracle_sieve, popcount,
+                      ref_trial_div_cpu, survivors)
 
 def test_golden_fixture_matches_oracle():
     n = 0
@@ -34,7 +21,7 @@ def test_golden_fixture_matches_oracle():
     assert n == 12
 
 
-@pytest.mark.skipif(ref_td() is None, reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.skipif(not have_ref_td(), reason="oracle/_ref not built (reference sources absent)")
 def test_oracle_equals_reference_cpu_random_triples():
     rng = random.Random(0x5EED)
     primes = numpy_primes(30000)
@@ -47,7 +34,7 @@ def test_oracle_equals_reference_cpu_random_triples():
         assert (ref == got).all(), (a, b, c)
 
 
-@pytest.mark.skipif(ref_td() is None, reason="oracle/_ref not built (reference sources absent)")
+@pytest.mark.skipif(not have_ref_td(), reason="oracle/_ref not built (reference sources absent)")
 def test_oracle_equals_reference_cpu_edge_exponents():
     # a-b-l and a-c-l crossing zero for l = 1..3; 2 and 5 in the list.
     primes = numpy_primes(5000)
@@ -144,3 +131,39 @@ def test_pipeline_fixture_survivors_match_oracle():
     assert len(surv) == fx["survivor_count"]
     packed = np.array([(k << 16) | j for k, j in surv], dtype="<u4")
     assert hashlib.sha256(packed.tobytes()).hexdigest() == fx["survivors_sha256"]
+
+
+def test_vanishing_power_of_ten_groups_follow_trial_div_cpu():
+    """p = 2 and 5 divide every power of ten above 10^0.  When a k's rev
+    exponent a-b-lk is 0 and a j digit group's exponent a-c-lj is positive,
+    rev(N) mod p does not depend on j inside that group, so exact
+    divisibility eliminates the whole group when it eliminates one pair.
+    The reference's GPU kernel marks those pairs (its flat scan,
+    trial_div.cu:97-115); its CPU sieve trial_div_cpu.c, the contract for
+    Phase-1 parity, solves each group through the inverse of the group's
+    power of ten and skips a group whose power vanishes mod p
+    (trial_div_cpu.c:62-63,73,82), so it marks none of them.  This build
+    follows trial_div_cpu.c; the test pins the divergence: at (a, b, c) =
+    (4, 3, 1), p = 2, exact divisibility of the rev formula eliminates 499
+    pairs and the CPU sieve 4 (the j = 100 group, whose exponent is 0).
+    No workload reaches such triples (c = 100, b > c)."""
+    a, b, c = 4, 3, 1
+    exact = set()
+    for k in range(1, 101):
+        for j in range(1, 101):
+            lk, lj = len(str(k)), len(str(j))
+            if a - b - lk < 0 or a - c - lj < 0:
+                continue
+            rev = 10 ** a - int(str(k)[::-1]) * 10 ** (a - b - lk) - \
+                int(str(j)[::-1]) * 10 ** (a - c - lj) - 1
+            n = 10 ** a - k * 10 ** b - j * 10 ** c - 1
+            assert n % 2 == 1
+            if rev % 2 == 0:
+                exact.add((k, j))
+    assert len(exact) == 499
+    got = oracle_sieve(np.array([2], np.uint32), a, b, c, threads=1)
+    marked = {(k, j) for k in range(1, 101) for j in range(1, 101) if bit_get(got, 100, k, j)}
+    assert marked == {(k, 100) for k in (2, 4, 6, 8)}
+    assert marked < exact
+    if have_ref_td():
+        assert (ref_trial_div_cpu(np.array([2], np.uint32), a, b, c) == got).all()
