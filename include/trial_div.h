@@ -95,6 +95,24 @@ int trial_div_launch_kj_ev(const uint32_t *d_primes, uint32_t n_primes,
  * 2 = more than 65536 list entries below 2^16 (a duplicated list). */
 int trial_div_check_error(void);
 
+/* trial_div_launch_kj that allocates nothing: no hipMalloc, hipFree, stream
+ * or event creation, on this call or any other.  No counterpart in the
+ * reference; it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Its
+ * scratch (small-prime list and setup table, control words, rev(j) decode
+ * table, row-kernel work items: 6.97 MB) is one static device array that
+ * each GPU the library runs on holds until the process ends; every kernel
+ * runs in order on `stream`.  Limits: K, J in 1..9999 (-1 and a message
+ * otherwise) and at most 65536 list entries below 2^16 (latched, reported
+ * by trial_div_check_error() as 2).  All calls on one GPU share the array:
+ * calls that overlap in time on one GPU (other streams, other host threads)
+ * race and must be serialised by the caller.  Same arguments, results and
+ * return codes as trial_div_launch_kj. */
+int trial_div_launch_kj_noalloc(const uint32_t *d_primes, uint32_t n_primes,
+                                long long a, long long b, long long c,
+                                uint32_t K, uint32_t J,
+                                uint32_t *d_bits, void *stream);
+
 /* Survivor extraction of worker.c:45-54 on the device: the pairs (k, j) of
  * a K x J bitmap (device pointer, layout as trial_div_gpu_kj) whose bit is
  * clear, in lexicographic order, written to d_out (device, cap entries) as

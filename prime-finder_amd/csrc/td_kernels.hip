@@ -39,6 +39,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -783,15 +784,20 @@ void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long
     // lane kernel (VALU-bound); both only OR into the bitmap.  The join
     // before returning keeps the shared scratch (table, items) from being
     // overwritten by the next call while the row kernel still reads it.
+    // Without an auxiliary stream (the allocation-free launch) both kernels
+    // run in order on st.
+    const hipStream_t rs = s.aux ? s.aux : st;
     if (a - c >= DJ) td_items_kernel<DJ><<<1, 1024, 0, st>>>(s.tbl, s.ctrl, J, s.items);
-    (void)hipEventRecord(s.fork, st);
-    (void)hipStreamWaitEvent(s.aux, s.fork, 0);
+    if (s.aux) {
+        (void)hipEventRecord(s.fork, st);
+        (void)hipStreamWaitEvent(s.aux, s.fork, 0);
+    }
     if (a - c >= DJ)
-        td_rows_bal_kernel<DJ><<<row_groups, TD_BLOCK, lds, s.aux>>>(
+        td_rows_bal_kernel<DJ><<<row_groups, TD_BLOCK, lds, rs>>>(
             s.tbl, s.ctrl, s.items, s.rdec + TD_RDEC_OFF[DJ], K, J, d_bits);
     else
-        td_rows_kernel<DJ><<<row_groups, TD_BLOCK, lds, s.aux>>>(s.tbl, s.ctrl, K, J, d_bits);
-    (void)hipEventRecord(s.join, s.aux);
+        td_rows_kernel<DJ><<<row_groups, TD_BLOCK, lds, rs>>>(s.tbl, s.ctrl, K, J, d_bits);
+    if (s.aux) (void)hipEventRecord(s.join, s.aux);
     if (ev0) (void)hipEventRecord(ev0, st);
     // One merged rev residue per k needs every j digit group's exponent
     // a-c-lj >= 0; otherwise track the groups separately.
@@ -802,8 +808,69 @@ void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long
         td_large_kernel<DJ, false><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
             d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits);
     if (ev1) (void)hipEventRecord(ev1, st);
-    (void)hipStreamWaitEvent(st, s.join, 0);
+    if (s.aux) (void)hipStreamWaitEvent(st, s.join, 0);
 }
+
+// ---- allocation-free launch (trial_div_launch_kj_noalloc) ----
+//
+// The same kernels with their scratch (small-prime list, setup table,
+// control words, rev(j) decode table, row-kernel items) in one file-scope
+// device array instead of hipMalloc'd buffers, and every kernel on the
+// caller's stream.  Each device holds its own copy of the array until the
+// process ends; its device address is looked up once per device with
+// hipGetSymbolAddress (the array's name is not a device address in host
+// code).  The control words are cleared by td_reset_kernel / td_items_kernel
+// on every call; the error latch is read and cleared by
+// trial_div_check_error().
+constexpr size_t TD_NA_SMALL = 0;
+constexpr size_t TD_NA_TBL   = TD_NA_SMALL + TD_SMALL_CAP;
+constexpr size_t TD_NA_CTRL  = TD_NA_TBL + (size_t)TD_SMALL_CAP * TD_TBL;
+constexpr size_t TD_NA_RDEC  = TD_NA_CTRL + 64;                      // 256-byte aligned
+constexpr size_t TD_NA_ITEMS = TD_NA_RDEC + ((TD_RDEC_SIZE * sizeof(uint16_t) + 255u) / 256u) * 64u;
+constexpr size_t TD_NA_WORDS = TD_NA_ITEMS + TD_ITEM_CAP;           // 6.97 MB
+static_assert(CT_WORDS <= 64, "control words fit their slot");
+
+static __device__ __align__(256) uint32_t scratch[TD_NA_WORDS];
+
+struct TdNoAlloc {
+    uint32_t *ptr = nullptr;     // device address of `scratch` on this device
+};
+TdNoAlloc g_noalloc[TD_MAX_DEV];
+std::mutex g_noalloc_mu;
+
+// Device address of the array on the current device (looked up and the
+// decode table built on the first call per device, on stream st).
+int td_noalloc_scratch(hipStream_t st, TdScratch *out)
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= TD_MAX_DEV) return -1;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    TdNoAlloc &na = g_noalloc[dev];
+    if (!na.ptr) {
+        uint32_t *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            fprintf(stderr, "trial_div_launch_kj_noalloc: hipGetSymbolAddress failed: %s\n",
+                    hipGetErrorString(e));
+            return -1;
+        }
+        td_rdec_kernel<<<(TD_RDEC_SIZE + 255u) / 256u, 256, 0, st>>>(
+            (uint16_t *)(ptr + TD_NA_RDEC));
+        if (hipGetLastError() != hipSuccess) return -1;
+        na.ptr = ptr;
+    }
+    *out = TdScratch{};
+    out->small = na.ptr + TD_NA_SMALL;
+    out->tbl = na.ptr + TD_NA_TBL;
+    out->ctrl = na.ptr + TD_NA_CTRL;
+    out->rdec = (uint16_t *)(na.ptr + TD_NA_RDEC);
+    out->items = na.ptr + TD_NA_ITEMS;
+    return 0;
+}
+
+int td_launch_with(const uint32_t *d_primes, uint32_t n, long long a, long long b, long long c,
+                   uint32_t K, uint32_t J, uint32_t *d_bits, hipStream_t st,
+                   hipEvent_t ev0, hipEvent_t ev1, TdScratch *s);
 
 int td_launch(const uint32_t *d_primes, uint32_t n, long long a, long long b, long long c,
               uint32_t K, uint32_t J, uint32_t *d_bits, hipStream_t st,
@@ -814,6 +881,13 @@ int td_launch(const uint32_t *d_primes, uint32_t n, long long a, long long b, lo
     if (!d_bits || (n && !d_primes)) return -1;
     TdScratch *s = nullptr;
     if (td_scratch(&s) != 0) return -1;
+    return td_launch_with(d_primes, n, a, b, c, K, J, d_bits, st, ev0, ev1, s);
+}
+
+int td_launch_with(const uint32_t *d_primes, uint32_t n, long long a, long long b, long long c,
+                   uint32_t K, uint32_t J, uint32_t *d_bits, hipStream_t st,
+                   hipEvent_t ev0, hipEvent_t ev1, TdScratch *s)
+{
     td_reset_kernel<<<1, 64, 0, st>>>(s->ctrl);
     if (n)
         td_split_kernel<<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
@@ -879,20 +953,38 @@ double td_secs(hipEvent_t x, hipEvent_t y)
     return (double)ms * 1e-3;
 }
 
+// Read and clear the error latch of one control block (device memory).
+int td_take_error(uint32_t *ctrl)
+{
+    uint32_t err = 0;
+    if (hipMemcpy(&err, ctrl + CT_ERROR, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    if (err) {
+        const uint32_t zero = 0;
+        if (hipMemcpy(ctrl + CT_ERROR, &zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess)
+            return -1;
+    }
+    return (int)err;
+}
+
 int td_check_error_code()
 {
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= TD_MAX_DEV) return -1;
     TdScratch &s = g_scratch[dev];
-    if (!s.ready) return 0;
-    uint32_t err = 0;
-    if (hipMemcpy(&err, s.ctrl + CT_ERROR, sizeof(err), hipMemcpyDeviceToHost) != hipSuccess) return -1;
-    if (err) {
-        const uint32_t zero = 0;
-        if (hipMemcpy(s.ctrl + CT_ERROR, &zero, sizeof(zero), hipMemcpyHostToDevice) != hipSuccess)
-            return -1;
+    return s.ready ? td_take_error(s.ctrl) : 0;
+}
+
+// The allocation-free launches' latch on the current device.
+int td_check_error_noalloc()
+{
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= TD_MAX_DEV) return -1;
+    uint32_t *ptr = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        ptr = g_noalloc[dev].ptr;
     }
-    return (int)err;
+    return ptr ? td_take_error(ptr + TD_NA_CTRL) : 0;
 }
 
 int td_run_host(const prime_list_t *pl, long long a, long long b, long long c,
@@ -1035,7 +1127,28 @@ extern "C" int trial_div_launch_kj_ev(const uint32_t *d_primes, uint32_t n_prime
 extern "C" int trial_div_check_error(void)
 {
     if (hipDeviceSynchronize() != hipSuccess) return -1;
-    return td_check_error_code();
+    const int e0 = td_check_error_code();
+    const int e1 = td_check_error_noalloc();
+    return e0 != 0 ? e0 : e1;
+}
+
+extern "C" int trial_div_launch_kj_noalloc(const uint32_t *d_primes, uint32_t n_primes,
+                                           long long a, long long b, long long c,
+                                           uint32_t K, uint32_t J, uint32_t *d_bits, void *stream)
+{
+    if (K < 1u || J < 1u || K > 9999u || J > 9999u) {
+        fprintf(stderr, "trial_div_launch_kj_noalloc: K = %u, J = %u outside 1..9999 "
+                        "(the scratch array is sized for that grid)\n", K, J);
+        return -1;
+    }
+    if (a < 0 || b < 0 || c < 0 || !d_bits || (n_primes && !d_primes)) {
+        fprintf(stderr, "trial_div_launch_kj_noalloc: invalid arguments\n");
+        return -1;
+    }
+    const hipStream_t st = (hipStream_t)stream;
+    TdScratch s;
+    if (td_noalloc_scratch(st, &s) != 0) return -1;
+    return td_launch_with(d_primes, n_primes, a, b, c, K, J, d_bits, st, nullptr, nullptr, &s);
 }
 
 void trial_div_launch(const uint32_t *d_primes, uint32_t n_primes, uint64_t a, uint64_t b,

@@ -48,6 +48,7 @@ def lib() -> C.CDLL:
                                        C.POINTER(GpuTiming)]),
         "trial_div_launch_kj": (C.c_int, [vp, u32, ll, ll, ll, u32, u32, vp, vp]),
         "trial_div_launch_kj_ev": (C.c_int, [vp, u32, ll, ll, ll, u32, u32, vp, vp, vp, vp]),
+        "trial_div_launch_kj_noalloc": (C.c_int, [vp, u32, ll, ll, ll, u32, u32, vp, vp]),
         "trial_div_check_error": (C.c_int, []),
         "trial_div_survivors": (C.c_int, [vp, u32, u32, vp, u32, C.POINTER(C.c_uint32), vp]),
         "cuda_get_device_count": (C.c_int, []),
