@@ -32,6 +32,7 @@
 //                          a-c >= D_j (see td_hits), so a k step costs two
 //                          residue updates and two compares.
 #include "pf_hip.h"
+#include "td_lattice.h"
 #include "../../include/trial_div.h"
 
 #include <stdlib.h>
@@ -70,33 +71,9 @@ constexpr uint32_t TD_CH       = 32;
 constexpr uint32_t TD_ITEM_CAP = 32768;   // > 2 * 6542 + 20000/32 * sum(1/p)
 enum { TI_N = 0, TI_REV = 1, TI_WHOLE = 2 };
 
-__device__ __forceinline__ uint32_t td_pow10(int e)
-{
-    // e in [0, 5]; a select chain keeps it in registers.
-    uint32_t v = 1u;
-    v = e >= 1 ? 10u : v;
-    v = e >= 2 ? 100u : v;
-    v = e >= 3 ? 1000u : v;
-    v = e >= 4 ? 10000u : v;
-    v = e >= 5 ? 100000u : v;
-    return v;
-}
-
 __device__ __forceinline__ int td_ndigits(uint32_t x)
 {
     return 1 + (x >= 10u) + (x >= 100u) + (x >= 1000u) + (x >= 10000u);
-}
-
-// Reverse the lowest `len` decimal digits of x, leading zeros included.
-__device__ __forceinline__ uint32_t td_reverse(uint32_t x, int len)
-{
-    uint32_t out = 0;
-    for (int i = 0; i < len; i++) {
-        uint32_t q = x / 10u;
-        out = out * 10u + (x - q * 10u);
-        x = q;
-    }
-    return out;
 }
 
 // x mod p for x < 2^32 with magic = floor(2^32 / p).
@@ -491,14 +468,7 @@ __global__ void td_rdec_kernel(uint16_t *rdec)
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= TD_RDEC_SIZE) return;
     const int d = i < 10u ? 1 : i < 110u ? 2 : i < 1110u ? 3 : 4;
-    uint32_t r = i - td_rdec_offset(d);
-    uint32_t j = 0;
-    if (r != 0u) {
-        int lj = d;
-        while (r % 10u == 0u) { r /= 10u; lj--; }
-        j = td_reverse(r, lj);
-    }
-    rdec[i] = (uint16_t)j;
+    rdec[i] = (uint16_t)td_rdec_value(i - td_rdec_offset(d), d);
 }
 
 template <int DJ, bool MERGED>
@@ -506,13 +476,13 @@ __global__ __launch_bounds__(TD_BLOCK)
 void td_large_kernel(const uint32_t *__restrict__ primes, uint32_t n,
                      long long a, long long b, long long c,
                      uint32_t K, uint32_t J, const uint16_t *__restrict__ rdec,
-                     uint32_t *__restrict__ bits)
+                     uint32_t *__restrict__ bits, uint32_t pmax)
 {
     constexpr int NR = MERGED ? 1 : DJ;     // rev residues tracked per k
     const uint32_t i = blockIdx.x * TD_BLOCK + threadIdx.x;
     if (i >= n) return;
     const uint32_t p = primes[i];
-    if (p < TD_SMALL_LIMIT || p >= TD_PMAX) return;
+    if (p < TD_SMALL_LIMIT || p >= pmax) return;
 
     // ---- per-prime setup (Montgomery, R = 2^32) ----
     const PfMont m = pf_mont_init(p);
@@ -708,6 +678,35 @@ void td_large_kernel(const uint32_t *__restrict__ primes, uint32_t n,
     }
 }
 
+// Lattice kernel: one lane per prime >= TD_LAT_MIN (merged j digit groups),
+// hits enumerated directly by td_lattice_prime (td_lattice.h) instead of the
+// K-step loop of td_large_kernel.
+#ifndef TD_LAT_MIN_LOG2
+#define TD_LAT_MIN_LOG2 21
+#endif
+constexpr uint32_t TD_LAT_MIN = 1u << TD_LAT_MIN_LOG2;
+static_assert(TD_LAT_MIN >= TD_SMALL_LIMIT, "the lattice path needs p > 10^4 >= J");
+
+struct TdDevMark {
+    uint32_t *bits;
+    __device__ __forceinline__ void operator()(uint32_t bit) const { td_gmark(bits, bit); }
+};
+
+template <int DJ>
+__global__ __launch_bounds__(TD_BLOCK)
+void td_lattice_kernel(const uint32_t *__restrict__ primes, uint32_t n,
+                       long long a, long long b, long long c,
+                       uint32_t K, uint32_t J, const uint16_t *__restrict__ rdec,
+                       uint32_t *__restrict__ bits, uint32_t pmin)
+{
+    const uint32_t i = blockIdx.x * TD_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t p = primes[i];
+    if (p < pmin || p >= TD_PMAX) return;
+    TdDevMark mark{bits};
+    td_lattice_prime<DJ>(p, a, b, c, K, J, rdec, mark);
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -801,12 +800,17 @@ void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long
     if (ev0) (void)hipEventRecord(ev0, st);
     // One merged rev residue per k needs every j digit group's exponent
     // a-c-lj >= 0; otherwise track the groups separately.
-    if (n && a - c >= DJ)
+    // primes >= TD_LAT_MIN enumerate their hits directly when the j digit
+    // groups merge; below, and for separate groups, the K-step lane kernel.
+    if (n && a - c >= DJ) {
         td_large_kernel<DJ, true><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
-            d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits);
-    else if (n)
+            d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits, TD_LAT_MIN);
+        td_lattice_kernel<DJ><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
+            d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits, TD_LAT_MIN);
+    } else if (n) {
         td_large_kernel<DJ, false><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
-            d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits);
+            d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits, TD_PMAX);
+    }
     if (ev1) (void)hipEventRecord(ev1, st);
     if (s.aux) (void)hipStreamWaitEvent(st, s.join, 0);
 }

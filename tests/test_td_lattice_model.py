@@ -1,0 +1,107 @@
+"""CPU model of the trial-division lattice kernel.
+
+The kernel's per-prime routine (td_lattice_prime in
+prime-finder_amd/csrc/td_lattice.h: the N condition and each digit class of
+the rev condition as "first t with (c0 + t sig) mod p < T", found by a
+continued-fraction descent and repeated from the next t) is compiled for the
+host with g++ (tests/model/td_lattice_model.cpp) and compared bit for bit
+with the oracle's restatement of trial_div_cpu.c on primes above 10^4:
+random triples with exponent edges, grids up to 9999, primes next to 2^16,
+2^21, 2^24 and 2^31.  A composite list entry must end (no hang)."""
+import ctypes as C
+import random
+import subprocess
+
+import numpy as np
+import pytest
+
+from _support import ROOT, nwords, numpy_primes, oracle_sieve
+
+u32p = C.POINTER(C.c_uint32)
+
+
+@pytest.fixture(scope="module")
+def model(tmp_path_factory):
+    out = tmp_path_factory.mktemp("latmodel") / "td_lattice_model.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-Wall",
+                    f"-I{ROOT / 'prime-finder_amd' / 'csrc'}",
+                    str(ROOT / "tests" / "model" / "td_lattice_model.cpp"), "-o", str(out)],
+                   check=True)
+    L = C.CDLL(str(out))
+    L.lattice_sieve.argtypes = [u32p, C.c_uint32, C.c_longlong, C.c_longlong, C.c_longlong,
+                                C.c_uint32, C.c_uint32, u32p]
+    L.lattice_sieve.restype = C.c_int
+    return L
+
+
+def run_model(L, primes, a, b, c, K, J):
+    p = np.ascontiguousarray(primes, dtype=np.uint32)
+    bits = np.zeros(nwords(K, J), np.uint32)
+    assert L.lattice_sieve(p.ctypes.data_as(u32p), len(p), a, b, c, K, J,
+                           bits.ctypes.data_as(u32p)) == 0
+    return bits
+
+
+def _isprime(n):
+    if n < 2 or n % 2 == 0:
+        return n == 2
+    d = 3
+    while d * d <= n:
+        if n % d == 0:
+            return False
+        d += 2
+    return True
+
+
+def _primes_near(x, count, rng):
+    out, v = [], x - rng.randrange(0, 5000)
+    while len(out) < count:
+        if _isprime(v) and 10 ** 4 < v < 2 ** 31:
+            out.append(v)
+        v += 1
+    return out
+
+
+@pytest.fixture(scope="module")
+def big_primes():
+    rng = random.Random(11)
+    allp = numpy_primes(3_000_000)
+    ps = list(allp[allp > 10 ** 4][::997])
+    for x in (1 << 16, 1 << 21, 1 << 24, 10 ** 8, (1 << 31) - 200):
+        ps += _primes_near(x, 6, rng)
+    ps += [2147483647, 2147483629, 99999989, 16777259, 2097143, 2097169]
+    return np.array(sorted(set(ps)), np.uint32)
+
+
+@pytest.mark.parametrize("K,J", [(4096, 4096), (100, 100), (9999, 37), (37, 9999), (1000, 999),
+                                 (150, 130), (1, 1), (9, 10)])
+def test_model_equals_oracle(model, big_primes, K, J):
+    rng = random.Random(K * 10007 + J)
+    dj = len(str(J))
+    for _ in range(4):
+        a = rng.choice([rng.randrange(dj + 1, 200), rng.randrange(50000, 100001)])
+        c = rng.randrange(0, a - dj + 1)
+        b = rng.choice([rng.randrange(0, a + 4), c + 1, a - 1, a - 4, a])
+        got = run_model(model, big_primes, a, b, c, K, J)
+        want = oracle_sieve(big_primes, a, b, c, K, J)
+        assert (got == want).all(), (a, b, c, K, J)
+
+
+def test_model_workload2_triple_sample(model):
+    """A workload triple at K = J = 4096 over primes of every size class of
+    the lane kernels (the lattice path takes p >= 2^21 on the device)."""
+    import json
+    from _support import GOLDEN
+    cfg = json.loads((GOLDEN / "configs.json").read_text())["2"]
+    rng = np.random.default_rng(3)
+    allp = numpy_primes(10 ** 8)
+    sample = np.sort(rng.choice(allp[allp > 10 ** 4], 600, replace=False)).astype(np.uint32)
+    for (a, b, c) in cfg["triples"][:3]:
+        got = run_model(model, sample, a, b, c, 4096, 4096)
+        assert (got == oracle_sieve(sample, a, b, c, 4096, 4096)).all(), (a, b, c)
+
+
+def test_composite_entries_terminate(model):
+    comps = np.array([3 * 5 * 7 * 11 * 13 * 17, 1000001, 2147483647 * 0 + 2147483645,
+                      10 ** 8 + 1, 65535 * 3], np.uint32)
+    run_model(model, comps, 60000, 30000, 100, 4096, 4096)
