@@ -89,6 +89,40 @@ int mr_run_ex(mr_ctx_t *ctx, const uint32_t *bases, int n_bases, int flags,
 int mr_run_per_slot(mr_ctx_t *ctx, const uint32_t *bases, uint8_t *pass_out,
                     uint32_t *residues_out);
 
+/* ---- Arithmetic on caller-owned device vectors ----------------------
+ * The device-pointer methods of the reference's BatchModCtx
+ * (batch_mod_ctx.cuh:141-161: modmul_batch, modsq_batch, check_passed,
+ * to_residue_batch, from_residue_batch) in this library's working form: the
+ * residue of x for candidate c is x R mod N_c with R = 2^(24 L_c) (L_c =
+ * that candidate's 24-bit limb count, at most the L of mr_ctx_info), stored
+ * as L_c little-endian 24-bit limbs, one per uint64_t, at V + c * ld
+ * (ld >= L; entries from L_c up to ld are written as zero).  The reference
+ * keeps 16-bit limbs one per uint64_t (Data64) with R = 2^(16 n_limbs), so
+ * a buffer of n_batch * n_limbs16 elements holds either form.  The _dev
+ * calls are asynchronous on `stream` (a hipStream_t, NULL = null stream)
+ * and use the context's scratch: one call at a time per context.
+ * Candidates decided without rounds (even N, N < 4) give zero vectors and
+ * passed = 0.  Return 0, -1 (invalid argument, or a value not below its
+ * modulus) or -2 (HIP error). */
+
+/* d_out = a b R^-1 mod N per candidate (Montgomery product). */
+int mr_ctx_modmul_dev(mr_ctx_t *ctx, const uint64_t *d_a, const uint64_t *d_b, uint64_t *d_out,
+                      int ld, void *stream);
+/* d_out = a^2 R^-1 mod N per candidate. */
+int mr_ctx_modsq_dev(mr_ctx_t *ctx, const uint64_t *d_a, uint64_t *d_out, int ld, void *stream);
+/* d_passed[c] = 1 if r == R mod N or r == (N-1) R mod N (the round's pass
+ * test in working form), else 0. */
+int mr_ctx_check_passed_dev(mr_ctx_t *ctx, const uint64_t *d_r, int ld, uint8_t *d_passed,
+                            void *stream);
+/* Host to host: x (16-bit limbs, one per uint64_t, [n_batch][n_limbs16],
+ * each x < N) -> its residue x R mod N in working form, out [n_batch][ld].
+ * Synchronous. */
+int mr_ctx_to_residue(mr_ctx_t *ctx, const uint64_t *x16, int n_limbs16, uint64_t *out, int ld);
+/* Device to host: residues d_x (working form) -> x R^-1 mod N as 16-bit
+ * limbs, out16 [n_batch][n_limbs16].  Synchronous. */
+int mr_ctx_from_residue_dev(mr_ctx_t *ctx, const uint64_t *d_x, int ld, uint64_t *out16,
+                            int n_limbs16);
+
 /* Per-context geometry: words per number, limbs L (24-bit), log2 of the
  * transform length, and s (N - 1 = 2^s d) per candidate (may be NULL). */
 int mr_ctx_info(const mr_ctx_t *ctx, int *n_words, int *limbs, int *log_len, int *s_out);
@@ -134,12 +168,20 @@ typedef struct {
     int64_t chunks;       /* squaring-chunk launches                       */
     int64_t admin_batches;
     int64_t squarings;    /* slot-steps issued (upper bound of work done)  */
-    int64_t hbm_bytes;    /* slot memory allocated                         */
+    int64_t hbm_bytes;    /* slot memory allocated (the pools' slots)      */
     int32_t max_active, n_pools;
+    int64_t job_bytes;    /* footprint of the admitted jobs (<= hbm_budget) */
+    int64_t peak_job_bytes;
+    int64_t reclaimed_pools; /* idle pools released to admit another size  */
 } mr_sched_stats_t;
 
 mr_sched_t *mr_sched_create(const uint32_t *bases, int n_bases, int device,
                             int max_active, int chunk, int64_t hbm_budget);
+/* Device bytes one job of a candidate with `bits` bits occupies while
+ * admitted (its slot: N, NTT(N), NTT(N'), residue, transform scratch), the
+ * footprint admission counts against hbm_budget; -1 beyond this build's
+ * sizes. */
+int64_t mr_sched_job_bytes(int64_t bits);
 /* Queue n candidates (validated first; -1 and nothing queued if any cannot
  * be formed or is too large). */
 int mr_sched_submit(mr_sched_t *s, const mr_candidate_t *cands, const int64_t *tags, int n);

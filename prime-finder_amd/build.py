@@ -2,9 +2,10 @@
 
 Every ``csrc/*.hip`` file is compiled by hipcc for gfx950 only, every
 ``csrc/*.c`` file by gcc, and the objects are linked into one shared library
-whose exported C symbols are declared in ``include/*.h``.  ``mr_cpp_check``
-(``csrc/mr_cpp_check.cpp``, a C++ caller of ``include/batch_mod_ctx.hpp``)
-is linked against it.
+whose exported C symbols are declared in ``include/*.h``.  The compiled
+callers of ``callers/`` (``worker_dropin``, ``bench_gpu_dropin``,
+``mr_cpp_check``: the reference's worker, bench and runner call sites) are
+linked against it.
 """
 from __future__ import annotations
 
@@ -18,6 +19,7 @@ from pathlib import Path
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
 CSRC = PKG / "csrc"
+CALLERS = PKG / "callers"
 OBJ = PKG / "_build"
 LIB = PKG / "lib" / "libpfamd.so"
 
@@ -64,10 +66,15 @@ def build(verbose: bool = False) -> Path:
     tmp = LIB.with_suffix(".so.tmp")
     _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *objs])
     shutil.move(str(tmp), str(LIB))
-    # C++ caller of include/batch_mod_ctx.hpp (the reference's runner surface)
-    _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}",
-          str(CSRC / "mr_cpp_check.cpp"), "-o", str(LIB.parent / "mr_cpp_check"),
-          f"-L{LIB.parent}", "-lpfamd", "-Wl,-rpath,$ORIGIN"])
+    # compiled callers of the drop-in boundary (callers/): the reference's
+    # worker (C), its tests/bench_gpu.cu (C++, trial_div_launch) and its
+    # runner surface (C++, include/batch_mod_ctx.hpp)
+    link = [f"-L{LIB.parent}", "-lpfamd", "-Wl,-rpath,$ORIGIN"]
+    _run(["gcc", "-O2", "-std=c11", "-Wall", "-Wextra", f"-I{ROOT / 'include'}",
+          str(CALLERS / "worker_dropin.c"), "-o", str(LIB.parent / "worker_dropin"), *link])
+    for name in ("bench_gpu_dropin", "mr_cpp_check"):
+        _run(["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}",
+              str(CALLERS / f"{name}.cpp"), "-o", str(LIB.parent / name), *link])
     return LIB
 
 

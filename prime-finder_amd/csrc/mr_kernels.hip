@@ -1003,6 +1003,107 @@ __global__ __launch_bounds__(THREADS) void debug_kernel(Dev D, Twiddles T, Const
     }
 }
 
+// ---- arithmetic on caller-owned device vectors (the reference runner's
+// BatchModCtx methods, batch_mod_ctx.cuh:141-161).  A vector of candidate c
+// is 24-bit limbs, one per uint64_t, at V + c * ld; a residue is x R mod N
+// with R = 2^(24 L_c).  Outputs are written with L_c limbs and zeros up to
+// ld.
+enum { AR_MUL = 0, AR_SQ = 1, AR_CHECK = 2, AR_R2 = 3, AR_TORES = 4, AR_FROMRES = 5 };
+
+template <int C>
+__device__ __forceinline__ void load_vec64(uint32_t (&v)[C], const uint64_t *src, int len, int off)
+{
+    const int p0 = tid_() * C;
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+        const int idx = p0 + i - off;
+        v[i] = (idx >= 0 && idx < len) ? (uint32_t)src[idx] & LIMB_MASK : 0u;
+    }
+}
+
+// dst[0, ld) = the positions [off, off + len) of v, then zeros.
+template <int C>
+__device__ __forceinline__ void store_vec64(const uint32_t (&v)[C], uint64_t *dst, int len, int off,
+                                            int ld)
+{
+    const int p0 = tid_() * C;
+#pragma unroll
+    for (int i = 0; i < C; i++) {
+        const int idx = p0 + i - off;
+        if (idx >= 0 && idx < len) dst[idx] = v[i];
+    }
+    for (int idx = len + (int)threadIdx.x; idx < ld; idx += THREADS) dst[idx] = 0u;
+}
+
+// op AR_MUL: out = A B R^-1 mod N; AR_SQ: A^2 R^-1; AR_CHECK: flag = (A ==
+// R mod N or A == (N-1) R mod N); AR_R2: r2 = R^2 mod N (as 2^(24 L) in
+// Montgomery form: left-to-right powering of 2, squarings and doublings);
+// AR_TORES: out = A R mod N (A < N, times r2); AR_FROMRES: out = A R^-1 mod
+// N.  Slots decided without rounds (even N, N < 4) get zeros and flag 0.
+template <int LOGM>
+__global__ __launch_bounds__(THREADS) void arith_kernel(Dev D, Twiddles T, Consts K, int op,
+                                                        const uint64_t *A, const uint64_t *B,
+                                                        int ld, uint64_t *out, uint8_t *flag,
+                                                        uint32_t *r2)
+{
+    constexpr int M = G<LOGM>::M, C = G<LOGM>::C;
+    __shared__ Smem<LOGM> sm;
+    const int c = blockIdx.x;
+    const int L = D.Ls[c];
+    uint32_t *scr = D.scr + (size_t)c * 2 * M;
+    uint32_t *fsave = D.ysv + (size_t)c * G<LOGM>::YSV + M;
+    uint32_t a[C];
+    if (D.pre[c] != 0) {
+#pragma unroll
+        for (int i = 0; i < C; i++) a[i] = 0u;
+        if (op == AR_CHECK) {
+            if (threadIdx.x == 0) flag[c] = 0;
+        } else if (op == AR_R2) {
+            store_vec<C>(a, r2 + (size_t)c * D.stride, L, 0);
+        } else {
+            store_vec64<C>(a, out + (size_t)c * ld, L, 0, ld);
+        }
+        return;
+    }
+    if (op == AR_CHECK) {
+        load_vec64<C>(a, A + (size_t)c * ld, L, 0);
+        const bool pass = vec_equal_mem<C>(a, D.one + (size_t)c * D.stride, L, 0, sm.sc) ||
+                          vec_equal_mem<C>(a, D.nm1 + (size_t)c * D.stride, L, 0, sm.sc);
+        if (threadIdx.x == 0) flag[c] = pass ? 1 : 0;
+        return;
+    }
+    if (op == AR_R2) {
+        const uint32_t *Nl = D.N + (size_t)c * D.stride;
+        load_vec<C>(a, D.one + (size_t)c * D.stride, L, L);   // Montgomery form of 1, high layout
+        const uint32_t e = 24u * (uint32_t)L;
+        for (int i = 31 - __clz((int)e); i >= 0; i--) {
+            montmul<LOGM>(sm, T, K, D, c, a, L, nullptr);
+            if ((e >> i) & 1u)
+                mulsmall_modN<C>(a, 2u, Nl, L, L, sm.X, sm.sc, D.Nh + (size_t)c * M);
+        }
+        store_vec<C>(a, r2 + (size_t)c * D.stride, L, L);
+        return;
+    }
+    if (op == AR_SQ) {
+        load_vec64<C>(a, A + (size_t)c * ld, L, L);           // high layout
+        montmul<LOGM>(sm, T, K, D, c, a, L, nullptr);
+    } else {
+        uint32_t b[C];
+        load_vec64<C>(a, A + (size_t)c * ld, L, 0);
+        if (op == AR_MUL) {
+            load_vec64<C>(b, B + (size_t)c * ld, L, 0);
+        } else if (op == AR_TORES) {
+            load_vec<C>(b, r2 + (size_t)c * D.stride, L, 0);
+        } else {
+#pragma unroll
+            for (int i = 0; i < C; i++) b[i] = (tid_() == 0 && i == 0) ? 1u : 0u;
+        }
+        to_fft<LOGM>(sm, T, K, b, L, scr, fsave);
+        montmul<LOGM>(sm, T, K, D, c, a, 0, scr);
+    }
+    store_vec64<C>(a, out + (size_t)c * ld, L, L, ld);
+}
+
 } // namespace mr
 
 // ------------------------------------------------------------------ host
@@ -1563,6 +1664,149 @@ extern "C" int mr_debug_mulsmall(mr_ctx_t *x, const uint32_t *a, uint32_t s, uin
 
 namespace {
 
+int arith_launch(mr_ctx *x, int op, const uint64_t *A, const uint64_t *B, int ld, uint64_t *out,
+                 uint8_t *flag, uint32_t *r2, hipStream_t st)
+{
+    MR_CHECK(hipSetDevice(x->device));
+    const int rc = dispatch_logm(x->logm, [&](auto lg) {
+        constexpr int LG = decltype(lg)::value;
+        arith_kernel<LG><<<x->n, THREADS, 0, st>>>(x->d, x->T, x->K, op, A, B, ld, out, flag, r2);
+        return 0;
+    });
+    if (rc) return rc;
+    MR_CHECK(hipGetLastError());
+    return 0;
+}
+
+// 16-bit limbs (one per uint64_t) <-> 24-bit limbs (one per uint64_t).
+void limbs16_to_24(const uint64_t *x16, int n16, uint64_t *x24, int n24)
+{
+    for (int i = 0; i < n24; i++) x24[i] = 0u;
+    for (int i = 0; i < n16; i++) {
+        const uint64_t v = x16[i] & 0xFFFFu;
+        const long bit = 16L * i;
+        const long w = bit / 24, sh = bit % 24;
+        if (w < n24) x24[w] |= (v << sh) & 0xFFFFFFu;
+        if (sh > 8 && w + 1 < n24) x24[w + 1] |= v >> (24 - sh);
+    }
+}
+
+void limbs24_to_16(const uint64_t *x24, int n24, uint64_t *x16, int n16)
+{
+    for (int i = 0; i < n16; i++) x16[i] = 0u;
+    for (int i = 0; i < n24; i++) {
+        const uint64_t v = x24[i] & 0xFFFFFFu;
+        const long bit = 24L * i;
+        for (long b = 0; b < 24; b += 8) {          // byte by byte
+            const long at = bit + b;
+            const long w = at / 16, sh = at % 16;
+            if (w < n16) x16[w] |= ((v >> b) & 0xFFu) << sh;
+        }
+    }
+}
+
+// a < b for little-endian 24-bit limb vectors of length n.
+bool limbs_less(const uint64_t *a, const uint32_t *b, int n)
+{
+    for (int i = n - 1; i >= 0; i--) {
+        if (a[i] != b[i]) return a[i] < b[i];
+    }
+    return false;
+}
+
+} // namespace
+
+extern "C" int mr_ctx_modmul_dev(mr_ctx_t *x, const uint64_t *d_a, const uint64_t *d_b,
+                                 uint64_t *d_out, int ld, void *stream)
+{
+    if (!x || !d_a || !d_b || !d_out || ld < x->L) return -1;
+    return arith_launch(x, AR_MUL, d_a, d_b, ld, d_out, nullptr, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int mr_ctx_modsq_dev(mr_ctx_t *x, const uint64_t *d_a, uint64_t *d_out, int ld,
+                                void *stream)
+{
+    if (!x || !d_a || !d_out || ld < x->L) return -1;
+    return arith_launch(x, AR_SQ, d_a, nullptr, ld, d_out, nullptr, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int mr_ctx_check_passed_dev(mr_ctx_t *x, const uint64_t *d_r, int ld, uint8_t *d_passed,
+                                       void *stream)
+{
+    if (!x || !d_r || !d_passed || ld < x->L) return -1;
+    return arith_launch(x, AR_CHECK, d_r, nullptr, ld, nullptr, d_passed, nullptr,
+                        (hipStream_t)stream);
+}
+
+extern "C" int mr_ctx_to_residue(mr_ctx_t *x, const uint64_t *x16, int n_limbs16, uint64_t *out,
+                                 int ld)
+{
+    if (!x || !x16 || !out || n_limbs16 < 1 || ld < x->L) return -1;
+    MR_CHECK(hipSetDevice(x->device));
+    const int n = x->n, L = x->L;
+    std::vector<uint64_t> h((size_t)n * ld, 0u);
+    std::vector<uint32_t> N((size_t)n * x->stride);
+    MR_CHECK(hipMemcpyAsync(N.data(), x->d.N, sizeof(uint32_t) * N.size(), hipMemcpyDeviceToHost, x->st));
+    MR_CHECK(hipStreamSynchronize(x->st));
+    for (int c = 0; c < n; c++) {
+        limbs16_to_24(x16 + (size_t)c * n_limbs16, n_limbs16, h.data() + (size_t)c * ld, ld);
+        for (int i = L; i < ld; i++)
+            if (h[(size_t)c * ld + i]) {
+                fprintf(stderr, "%s: to_residue: value %d is not below its modulus\n", kWho, c);
+                return -1;
+            }
+        if (!limbs_less(h.data() + (size_t)c * ld, N.data() + (size_t)c * x->stride, L)) {
+            fprintf(stderr, "%s: to_residue: value %d is not below its modulus\n", kWho, c);
+            return -1;
+        }
+    }
+    uint64_t *d_in = nullptr, *d_out = nullptr;
+    uint32_t *d_r2 = nullptr;
+    int rc = 0;
+    hipError_t e;
+    if ((e = hipMalloc(&d_in, sizeof(uint64_t) * h.size())) != hipSuccess ||
+        (e = hipMalloc(&d_out, sizeof(uint64_t) * h.size())) != hipSuccess ||
+        (e = hipMalloc(&d_r2, sizeof(uint32_t) * (size_t)n * x->stride)) != hipSuccess)
+        rc = hip_fail(e, "to_residue alloc");
+    if (rc == 0 && (e = hipMemcpyAsync(d_in, h.data(), sizeof(uint64_t) * h.size(),
+                                       hipMemcpyHostToDevice, x->st)) != hipSuccess)
+        rc = hip_fail(e, "to_residue upload");
+    if (rc == 0) rc = arith_launch(x, AR_R2, nullptr, nullptr, ld, nullptr, nullptr, d_r2, x->st);
+    if (rc == 0) rc = arith_launch(x, AR_TORES, d_in, nullptr, ld, d_out, nullptr, d_r2, x->st);
+    if (rc == 0 && ((e = hipMemcpyAsync(out, d_out, sizeof(uint64_t) * h.size(),
+                                        hipMemcpyDeviceToHost, x->st)) != hipSuccess ||
+                    (e = hipStreamSynchronize(x->st)) != hipSuccess))
+        rc = hip_fail(e, "to_residue");
+    (void)hipFree(d_in);
+    (void)hipFree(d_out);
+    (void)hipFree(d_r2);
+    return rc;
+}
+
+extern "C" int mr_ctx_from_residue_dev(mr_ctx_t *x, const uint64_t *d_x, int ld, uint64_t *out16,
+                                       int n_limbs16)
+{
+    if (!x || !d_x || !out16 || n_limbs16 < 1 || ld < x->L) return -1;
+    MR_CHECK(hipSetDevice(x->device));
+    const int n = x->n;
+    std::vector<uint64_t> h((size_t)n * ld);
+    uint64_t *d_out = nullptr;
+    hipError_t e = hipMalloc(&d_out, sizeof(uint64_t) * h.size());
+    if (e != hipSuccess) return hip_fail(e, "from_residue alloc");
+    int rc = arith_launch(x, AR_FROMRES, d_x, nullptr, ld, d_out, nullptr, nullptr, x->st);
+    if (rc == 0 && ((e = hipMemcpyAsync(h.data(), d_out, sizeof(uint64_t) * h.size(),
+                                        hipMemcpyDeviceToHost, x->st)) != hipSuccess ||
+                    (e = hipStreamSynchronize(x->st)) != hipSuccess))
+        rc = hip_fail(e, "from_residue");
+    (void)hipFree(d_out);
+    if (rc == 0)
+        for (int c = 0; c < n; c++)
+            limbs24_to_16(h.data() + (size_t)c * ld, ld, out16 + (size_t)c * n_limbs16, n_limbs16);
+    return rc;
+}
+
+namespace {
+
 int ndig_u(uint64_t v)
 {
     int n = 1;
@@ -1790,21 +2034,27 @@ struct Pool {
     std::vector<int> req;          // slot -> request index
     std::vector<int> run_list, adm_list;
     int adm_n_end = 0;
+    int busy_slots = 0;            // slots holding an admitted job (ADM, RUN, END)
     bool run_busy = false, adm_busy = false;
-    std::vector<int> pending;      // request indices, FIFO
-    size_t pend_head = 0;
 };
 
 } // namespace
 
 struct mr_sched {
     int device = 0, n_bases = 0, max_active = 0, chunk = 256;
-    int64_t budget = 0, hbm = 0;
+    int64_t budget = 0;
+    int64_t hbm = 0;               // device memory held by the pools (all slots)
+    int64_t job_bytes = 0;         // footprint of the admitted jobs (their slots)
+    int64_t peak_job_bytes = 0;
+    int64_t reclaimed = 0;         // idle pools released to make room
     std::vector<uint32_t> bases;
     std::vector<Req> reqs;
     std::vector<mr_result_t> done;
     size_t done_head = 0;
     Pool *pools[MAX_LOGM + 1] = {};
+    // queued request indices per transform bucket, FIFO
+    std::vector<int> queue[MAX_LOGM + 1];
+    size_t qhead[MAX_LOGM + 1] = {};
     int active = 0;
     int64_t n_done = 0, chunks = 0, admins = 0, squarings = 0;
     bool failed = false;
@@ -1835,10 +2085,7 @@ int pool_new(mr_sched *S, int logm, Pool **out)
     int64_t room = (S->budget - S->hbm) / per;
     int n = S->max_active;
     if (room < n) n = (int)room;
-    if (n < 1) {
-        fprintf(stderr, "%s: HBM budget exhausted (%lld bytes in use)\n", kWho, (long long)S->hbm);
-        return -1;
-    }
+    if (n < 1) return 1;                 // no room under the budget now
     Pool *p = new (std::nothrow) Pool;
     if (!p) return -2;
     p->logm = logm;
@@ -1886,6 +2133,40 @@ int pool_new(mr_sched *S, int logm, Pool **out)
     p->req.assign(n, -1);
     S->hbm += per * n;
     *out = p;
+    return 0;
+}
+
+// Release pools that hold no admitted job while another bucket waits for
+// room under the budget (their memory goes back to the budget).
+void reclaim_idle(mr_sched *S, int except)
+{
+    for (int lg = 0; lg <= MAX_LOGM; lg++) {
+        Pool *&p = S->pools[lg];
+        if (!p || lg == except || p->busy_slots || p->run_busy || p->adm_busy) continue;
+        if (S->qhead[lg] < S->queue[lg].size()) continue;      // its own jobs are waiting
+        S->hbm -= slot_bytes(lg) * p->n;
+        pool_free(p);
+        p = nullptr;
+        S->reclaimed++;
+    }
+}
+
+// A pool for every bucket with queued jobs, as the budget allows: a pool
+// takes min(max_active, room / slot bytes) slots; a bucket that finds no
+// room releases idle pools of other buckets first, else its jobs wait in
+// the queue until room frees up.
+int ensure_pools(mr_sched *S)
+{
+    for (int lg = 0; lg <= MAX_LOGM; lg++) {
+        if (S->pools[lg] || S->qhead[lg] >= S->queue[lg].size()) continue;
+        int rc = pool_new(S, lg, &S->pools[lg]);
+        if (rc == 1) {
+            reclaim_idle(S, lg);
+            rc = pool_new(S, lg, &S->pools[lg]);
+        }
+        if (rc == 1) { S->pools[lg] = nullptr; continue; }
+        if (rc) { S->pools[lg] = nullptr; return rc; }
+    }
     return 0;
 }
 
@@ -1972,9 +2253,15 @@ int launch_admin(mr_sched *S, Pool *p)
         if (p->state[c] == SL_END) p->adm_list.push_back(c);
     const int n_end = (int)p->adm_list.size();
     int n_res = 0;          // restored candidates among the admitted
-    for (int c = 0; c < p->n && p->pend_head < p->pending.size() && S->active < S->max_active; c++) {
+    std::vector<int> &queue = S->queue[p->logm];
+    size_t &qh = S->qhead[p->logm];
+    const int64_t per = slot_bytes(p->logm);
+    // a job is admitted when a slot of its bucket is free, the active cap
+    // allows, and its footprint (its slot's device state) fits the budget
+    for (int c = 0; c < p->n && qh < queue.size() && S->active < S->max_active &&
+                    S->job_bytes + per <= S->budget; c++) {
         if (p->state[c] != SL_FREE) continue;
-        const int r = p->pending[p->pend_head++];
+        const int r = queue[qh++];
         Req &q = S->reqs[r];
         const int i = (int)p->adm_list.size() - n_end;
         memcpy(p->h_terms + (size_t)i * TMAX * 2, q.terms, sizeof(q.terms));
@@ -1995,7 +2282,11 @@ int launch_admin(mr_sched *S, Pool *p)
         p->req[c] = r;
         p->adm_list.push_back(c);
         S->active++;
+        S->job_bytes += per;
+        S->peak_job_bytes = std::max(S->peak_job_bytes, S->job_bytes);
+        p->busy_slots++;
     }
+    if (qh == queue.size()) { queue.clear(); qh = 0; }
     const int n_all = (int)p->adm_list.size(), n_new = n_all - n_end;
     if (n_all == 0) return 0;
     for (int c : p->adm_list) p->state[c] = SL_ADM;
@@ -2068,6 +2359,8 @@ int admin_done(mr_sched *S, Pool *p)
             p->state[c] = SL_FREE;
             p->req[c] = -1;
             S->active--;
+            S->job_bytes -= slot_bytes(p->logm);
+            p->busy_slots--;
         } else {
             p->state[c] = SL_RUN;
         }
@@ -2138,19 +2431,23 @@ bool make_req(const mr_candidate_t &cd, int64_t tag, Req &q)
     return true;
 }
 
-// Queue validated requests (pools created on demand).
+// Queue validated requests in their bucket (pools are created by the pump
+// as the budget allows).  A job whose footprint alone exceeds the budget
+// could never be admitted and is rejected with the whole call.
 int enqueue(mr_sched *S, std::vector<Req> &add)
 {
     const double now = mono_now();
     if (hipSetDevice(S->device) != hipSuccess) return -2;
-    for (Req &q : add) {
-        Pool *&p = S->pools[q.logm];
-        if (!p) {
-            int rc = pool_new(S, q.logm, &p);
-            if (rc) { p = nullptr; return rc; }
+    for (const Req &q : add)
+        if (slot_bytes(q.logm) > S->budget) {
+            fprintf(stderr, "%s: a candidate needs %lld bytes of device state, more than the "
+                            "HBM budget of %lld\n", kWho, (long long)slot_bytes(q.logm),
+                    (long long)S->budget);
+            return -1;
         }
+    for (Req &q : add) {
         q.t_submit = now;
-        p->pending.push_back((int)S->reqs.size());
+        S->queue[q.logm].push_back((int)S->reqs.size());
         S->reqs.push_back(std::move(q));
     }
     return 0;
@@ -2203,8 +2500,10 @@ int sched_checkpoint(mr_sched *S, const char *path)
                                hipMemcpyDeviceToHost));
             rows.push_back(std::move(w));
         }
-        for (size_t i = p->pend_head; i < p->pending.size(); i++) rows.push_back(Row{p->pending[i], -1, 0, {}});
     }
+    for (int lg = 0; lg <= MAX_LOGM; lg++)
+        for (size_t i = S->qhead[lg]; i < S->queue[lg].size(); i++)
+            rows.push_back(Row{S->queue[lg][i], -1, 0, {}});
     std::sort(rows.begin(), rows.end(), [](const Row &x, const Row &y) { return x.r < y.r; });
     // 3. the file (written whole, then renamed over path)
     const std::string tmp = std::string(path) + ".tmp";
@@ -2332,15 +2631,18 @@ extern "C" int mr_sched_pump(mr_sched_t *S, int block)
                 else if (e != hipErrorNotReady) return fail(hip_fail(e, "admission"));
             }
         }
-        // 2. launch: admissions oldest-first over the pools, then chunks
+        // 2. pools for the buckets with queued jobs (as the budget allows);
+        //    launch admissions oldest-first over the pools, then chunks
+        { int rc = ensure_pools(S); if (rc) return fail(rc); }
         Pool *order[MAX_LOGM + 1];
         int np = 0;
         for (Pool *p : S->pools) if (p) order[np++] = p;
         for (int a = 1; a < np; a++)       // by submit time of the head request
             for (int b = a; b > 0; b--) {
                 auto head = [&](Pool *p) {
-                    return p->pend_head < p->pending.size() ? S->reqs[p->pending[p->pend_head]].t_submit
-                                                            : 1e300;
+                    const int lg = p->logm;
+                    return S->qhead[lg] < S->queue[lg].size()
+                               ? S->reqs[S->queue[lg][S->qhead[lg]]].t_submit : 1e300;
                 };
                 if (head(order[b]) < head(order[b - 1])) std::swap(order[b], order[b - 1]);
                 else break;
@@ -2396,7 +2698,19 @@ extern "C" int mr_sched_stats(const mr_sched_t *S, mr_sched_stats_t *o)
     int np = 0;
     for (Pool *p : S->pools) np += p != nullptr;
     o->n_pools = np;
+    o->job_bytes = S->job_bytes;
+    o->peak_job_bytes = S->peak_job_bytes;
+    o->reclaimed_pools = S->reclaimed;
     return 0;
+}
+
+extern "C" int64_t mr_sched_job_bytes(int64_t bits)
+{
+    if (bits < 1) return -1;
+    int L, logm;
+    geometry((long)bits, MIN_LOGM, &L, &logm);
+    if (logm > MAX_LOGM || L > MAX_LIMBS) return -1;
+    return slot_bytes(logm);
 }
 
 extern "C" void mr_sched_destroy(mr_sched_t *S)

@@ -232,9 +232,15 @@ def _mr_sigs(L):
         "mr_debug_mulsmall": (i32, [vp, u32p, u32, u32p]),
         "mr_time_squarings": (i32, [vp, u32, i32, C.POINTER(C.c_float)]),
         "mr_set_min_log_len": (i32, [i32]),
+        "mr_ctx_modmul_dev": (i32, [vp, vp, vp, vp, i32, vp]),
+        "mr_ctx_modsq_dev": (i32, [vp, vp, vp, i32, vp]),
+        "mr_ctx_check_passed_dev": (i32, [vp, vp, i32, vp, vp]),
+        "mr_ctx_to_residue": (i32, [vp, C.POINTER(C.c_uint64), i32, C.POINTER(C.c_uint64), i32]),
+        "mr_ctx_from_residue_dev": (i32, [vp, vp, i32, C.POINTER(C.c_uint64), i32]),
         "miller_rabin_test": (i32, [C.c_longlong, C.c_longlong, C.c_longlong, i32, i32,
                                     C.POINTER(MrTiming)]),
         "mr_sched_create": (vp, [C.POINTER(u32), i32, i32, i32, i32, C.c_int64]),
+        "mr_sched_job_bytes": (C.c_int64, [C.c_int64]),
         "mr_sched_submit": (i32, [vp, C.POINTER(MrCandidate), C.POINTER(C.c_int64), i32]),
         "mr_sched_pump": (i32, [vp, i32]),
         "mr_sched_results": (i32, [vp, C.POINTER(MrResult), i32]),
@@ -272,7 +278,9 @@ class MrSchedStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("pending", C.c_int64),
                 ("active", C.c_int64), ("chunks", C.c_int64), ("admin_batches", C.c_int64),
                 ("squarings", C.c_int64), ("hbm_bytes", C.c_int64),
-                ("max_active", C.c_int32), ("n_pools", C.c_int32)]
+                ("max_active", C.c_int32), ("n_pools", C.c_int32),
+                ("job_bytes", C.c_int64), ("peak_job_bytes", C.c_int64),
+                ("reclaimed_pools", C.c_int64)]
 
 
 def _cands(cands):

@@ -274,3 +274,17 @@ def sprp_python(N: int, base: int):
         if x == 1:
             break
     return False, r, s
+
+
+def write_gap_file(path, primes):
+    """A prime list in the reference's gap-encoded file format
+    (gen_primes.c:98-127): u32 count, u32 first prime, then one byte per
+    gap, a gap above 255 written as 0 and a little-endian u16."""
+    import struct
+    out = bytearray(struct.pack("<I", len(primes)))
+    if len(primes):
+        out += struct.pack("<I", int(primes[0]))
+    for i in range(1, len(primes)):
+        gap = int(primes[i]) - int(primes[i - 1])
+        out += bytes([gap]) if gap <= 255 else bytes([0, gap & 0xFF, gap >> 8])
+    path.write_bytes(bytes(out))

@@ -9,7 +9,7 @@ import subprocess
 import numpy as np
 import pytest
 
-from _support import PKG, ROOT, numpy_primes
+from _support import PKG, ROOT, numpy_primes, write_gap_file
 
 INCLUDE = ROOT / "include"
 
@@ -73,17 +73,6 @@ def test_prime_sieve_1e8_count(pf):
     assert len(p) == 5761455 and p[-1] == 99999989
 
 
-def _write_gap_file(path, primes):
-    """The reference's format (gen_primes.c:98-127)."""
-    out = bytearray(struct.pack("<I", len(primes)))
-    if len(primes):
-        out += struct.pack("<I", int(primes[0]))
-    for i in range(1, len(primes)):
-        gap = int(primes[i]) - int(primes[i - 1])
-        out += bytes([gap]) if gap <= 255 else bytes([0, gap & 0xFF, gap >> 8])
-    path.write_bytes(bytes(out))
-
-
 def test_prime_list_load_reference_format(pf, tmp_path):
     L = pf.lib()
     primes = numpy_primes(200000)[1:]                  # 2 excluded, as gen_primes writes
@@ -93,7 +82,7 @@ def test_prime_list_load_reference_format(pf, tmp_path):
     primes = np.concatenate([primes, np.array([last + 282, last + 282 + 65535, last + 65817 + 256],
                                               np.uint32)])
     f = tmp_path / "p.bin"
-    _write_gap_file(f, primes)
+    write_gap_file(f, primes)
     pl = pf.PrimeList()
     assert L.prime_list_load(C.byref(pl), str(f).encode()) == 0
     got = np.ctypeslib.as_array(pl.primes, shape=(pl.count,)).copy()

@@ -92,3 +92,50 @@ def test_cpp_runner_surface(gpu):
     exp = [int(all(gmp_sprp(n, w)[0] for w in witnesses)) for n in nums]
     assert got == exp
     assert exp[3] == 0          # 3215031751 fails at 11
+
+
+def _cpp(mode, text):
+    import subprocess
+    from _support import PKG
+    exe = PKG / "lib" / "mr_cpp_check"
+    assert exe.exists(), "mr_cpp_check not built"
+    r = subprocess.run([str(exe), mode], input=text, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r.stdout
+
+
+def test_cpp_runner_arithmetic(gpu):
+    """BatchModCtx's device-pointer arithmetic (batch_mod_ctx.cuh:141-155) from a
+    compiled C++ caller, as correctness_tests.cuh:217-218,318,353,387 use it:
+    to_residue_batch, modmul_batch, modsq_batch, check_passed and
+    from_residue_batch give x y mod N and x^2 mod N (Python integers), and
+    check_passed accepts the working forms of 1 and N - 1 only."""
+    import random
+    rng = random.Random(44)
+    nums = [2 ** 521 - 1, 2 ** 1279 - 1, gmp_build(1000, 500, 100, 32, 5, 0),
+            gmp_build(3000, 1200, 100, 77, 41, 1), gmp_build(6000, 2500, 300, 4001, 17, 0)]
+    nums = [n | 1 for n in nums]
+    n_limbs = max((n.bit_length() + 15) // 16 for n in nums)
+    xs = [rng.randrange(2, n) for n in nums]
+    xs[1] = nums[1] - 1                       # N - 1 passes check_passed
+    xs[2] = 1                                 # and so does 1
+    ys = [rng.randrange(1, n) for n in nums]
+    text = f"{n_limbs} {len(nums)}\n" + " ".join(
+        str(v) for group in (nums, xs, ys) for n in group for v in _limbs16(n, n_limbs))
+    out = _cpp("arith", text)
+    rows = [l.split() for l in out.splitlines() if l and l[0].isdigit()]
+    assert len(rows) == len(nums)
+    for (i, xy, sq, p1, px), n, x, y in zip(rows, nums, xs, ys):
+        assert int(xy, 16) == x * y % n, i
+        assert int(sq, 16) == x * x % n, i
+        assert p1 == "1"
+        assert px == ("1" if x in (1, n - 1) else "0"), i
+
+
+def test_cpp_runner_rejects_wrong_s_and_d(gpu):
+    """gpu_miller_rabin with an s or an exp_all that are not those of N throws
+    (the reference would silently test another exponent)."""
+    nums = [2 ** 521 - 1, gmp_build(1000, 500, 100, 32, 5, 0)]
+    n_limbs = max((n.bit_length() + 15) // 16 for n in nums)
+    text = f"{n_limbs} {len(nums)}\n" + " ".join(str(x) for n in nums for x in _limbs16(n, n_limbs))
+    assert _cpp("badargs", text).strip() == "ok"

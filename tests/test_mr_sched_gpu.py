@@ -201,3 +201,78 @@ def test_checkpoint_and_resume(gpu, tmp_path):
     with pytest.raises(ValueError):
         s4.restore(tmp_path / "missing.ckpt")
     s4.close()
+
+
+def _drain(s, watch):
+    """Pump until done, sampling the active count after every pump."""
+    res, peak = [], 0
+    while True:
+        left = s.pump(True)
+        res += s.results()
+        peak = max(peak, s.stats().active)
+        watch.append(peak)
+        if left == 0:
+            return res + s.results(), peak
+
+
+def test_admission_by_job_footprint(gpu):
+    """A budget that holds three jobs of a size: at most three are admitted at
+    once and the others wait in the queue (their slots' device state is
+    what admission counts, mr_sched_job_bytes); every verdict and round
+    count equals GMP's; the admitted footprint never exceeds the budget."""
+    rng = random.Random(31)
+    cands = mixed_candidates(rng, 1500, 9) + PRIMES_1000[:2]
+    bits = int(1500 * 3.3219280948873623) + 2
+    per = gpu.lib().mr_sched_job_bytes(bits)
+    assert per > 0
+    budget = 3 * per + per // 2
+    s = gpu.MrScheduler(BASES, hbm_budget=budget)
+    s.submit(cands, tags=list(range(len(cands))))
+    watch = []
+    res, peak = _drain(s, watch)
+    st = s.stats()
+    say("budget", budget, "per job", per, "peak active", peak, "peak bytes", st.peak_job_bytes)
+    assert peak <= 3 and st.peak_job_bytes <= budget and st.hbm_bytes <= budget
+    assert st.completed == len(cands) and st.job_bytes == 0
+    got = {r.tag: (r.verdict, r.rounds) for r in res}
+    assert [got[i] for i in range(len(cands))] == [expected(cd) for cd in cands]
+    s.close()
+
+
+def test_admission_rejects_jobs_larger_than_the_budget(gpu):
+    """A budget below one 8 000-digit job (2^12 transforms): that job is
+    rejected at submission, a 1 000-digit job (2^11, a smaller footprint)
+    is admitted and decided."""
+    per = gpu.lib().mr_sched_job_bytes(int(8000 * 3.3219280948873623) + 2)
+    assert gpu.lib().mr_sched_job_bytes(int(1000 * 3.3219280948873623) + 2) < per
+    s = gpu.MrScheduler(BASES, hbm_budget=per - 1)
+    with pytest.raises(ValueError):
+        s.submit([(8000, 2500, 300, 4001, 17, 0)])
+    # a smaller job still fits and runs
+    s.submit([PRIMES_1000[0]], tags=[0])
+    res = s.run_all()
+    assert [(r.verdict, r.rounds) for r in res] == [(1, 4)]
+    assert gpu.lib().mr_sched_job_bytes(10 ** 7) == -1          # beyond this build's sizes
+    s.close()
+
+
+def test_admission_tight_budget_two_sizes(gpu):
+    """A budget of two jobs of the larger size, jobs of two transform
+    lengths: the first size's pool is released once idle so the other size
+    can be admitted (no stall), and all verdicts are right."""
+    rng = random.Random(32)
+    small = mixed_candidates(rng, 1200, 4)
+    large = mixed_candidates(rng, 8000, 3) + [PRIMES_1000[1]]     # 2^12 and 2^11 transforms
+    per_large = gpu.lib().mr_sched_job_bytes(int(8000 * 3.3219280948873623) + 2)
+    s = gpu.MrScheduler(BASES, hbm_budget=2 * per_large)
+    cands = small + large
+    s.submit(cands, tags=list(range(len(cands))))
+    res, _ = _drain(s, [])
+    st = s.stats()
+    say("two sizes: pools", st.n_pools, "reclaimed", st.reclaimed_pools, "peak bytes",
+        st.peak_job_bytes)
+    assert st.peak_job_bytes <= 2 * per_large and st.hbm_bytes <= 2 * per_large
+    assert st.reclaimed_pools >= 1
+    got = {r.tag: (r.verdict, r.rounds) for r in res}
+    assert [got[i] for i in range(len(cands))] == [expected(cd) for cd in cands]
+    s.close()
