@@ -60,28 +60,154 @@ def _load(name, path):
     return mod
 
 
-def cpu_baseline(primes: np.ndarray, triple, budget_s: float) -> dict:
-    """Single-thread CPU restatement of trial_div_cpu.c on a prime sample."""
+_CPU_PRIMES = None      # set in the parent before the fork pool starts
+
+
+def cpu_share() -> int:
+    """Host cores this run may use: the process's affinity set, capped at
+    16 (one GPU's share of a shared box)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _td_cpu_worker(job):
+    """One process: the oracle (single-threaded restatement of
+    trial_div_cpu.c) on every stride-th prime of the whole list (small
+    primes included), sized to ~budget_s after a calibration probe."""
+    (a, b, c), offset, budget_s = job
     sys.path.insert(0, str(ROOT / "tests"))
     from _support import oracle_sieve
-    a, b, c = triple
-    big = primes[primes >= SMALL_LIMIT]
-    # calibrate on a small slice, then size the sample for ~budget_s
-    probe = big[:: max(1, len(big) // 200)][:200]
+    primes = _CPU_PRIMES
+    probe = primes[offset:: max(1, len(primes) // 300)][:300]
     t0 = time.perf_counter()
     oracle_sieve(probe, a, b, c, K, J, threads=1)
     per_prime = (time.perf_counter() - t0) / len(probe)
-    n = int(max(50, min(len(big), budget_s / max(per_prime, 1e-9))))
-    stride = max(1, len(big) // n)
-    sample = big[::stride][:n]
+    n = int(max(50, min(len(primes), budget_s / max(per_prime, 1e-9))))
+    stride = max(1, len(primes) // n)
+    sample = primes[offset % stride::stride][:n]
     t0 = time.perf_counter()
     oracle_sieve(sample, a, b, c, K, J, threads=1)
     dt = time.perf_counter() - t0
-    return {"value": K * J * len(sample) / dt, "unit": "pair*primes/s", "cores": 1,
-            "kind": "port",
-            "sample": (f"triple (a,b,c)=({a},{b},{c}), K=J={K}, {len(sample)} primes "
-                       f"(every {stride}th prime >= 2^16 below 10^8), {dt:.1f} s on 1 core; "
-                       f"oracle/td_oracle.c restating trial_div_cpu.c")}
+    return len(sample), stride, dt
+
+
+def _mr_cpu_worker(job):
+    """One process: GMP mpz_powm of base 2 over the top sample_bits bits of
+    d for candidates of the workload in turn (~budget_s), scaled to the
+    whole exponent: tests/hour of this core."""
+    cands, sample_bits, budget_s = job
+    sys.path.insert(0, str(ROOT / "tests"))
+    from _support import gmp_build, gmp_powm
+    spent, scaled, done, bits = 0.0, 0.0, 0, 0
+    for cd in cands:
+        n = gmp_build(*cd)
+        d = n - 1
+        while d % 2 == 0:
+            d //= 2
+        bits = d.bit_length()
+        e = d >> max(0, bits - sample_bits)
+        t0 = time.perf_counter()
+        gmp_powm(2, e, n)
+        dt = time.perf_counter() - t0
+        spent += dt
+        scaled += dt * bits / sample_bits
+        done += 1
+        if spent >= budget_s:
+            break
+    return done, scaled, spent, bits
+
+
+def _bpsw_cpu_worker(job):
+    """One process: mpz_probab_prime_p(N, 5) (what the reference worker runs,
+    miller_rabin.c:91) and a full base-2 mpz_powm on the same N."""
+    cd = job
+    sys.path.insert(0, str(ROOT / "tests"))
+    import math
+    from _support import gmp_build, gmp_powm, gmp_probab_prime, numpy_primes
+    # a Phase-2 input survived trial division: take the first k from cd's
+    # on whose N has no prime factor below 10^5, so BPSW runs in full
+    small = math.prod(int(q) for q in numpy_primes(100000))
+    a, b, c, k, j, w = cd
+    while True:
+        n = gmp_build(a, b, c, k, j, w)
+        if math.gcd(n, small) == 1:
+            break
+        k += 1
+    t0 = time.perf_counter()
+    gmp_probab_prime(n, 5)
+    t_bpsw = time.perf_counter() - t0
+    d = n - 1
+    while d % 2 == 0:
+        d //= 2
+    t0 = time.perf_counter()
+    gmp_powm(2, d, n)
+    return t_bpsw, time.perf_counter() - t0
+
+
+def cpu_baselines(budget_s: float) -> dict:
+    """The reference's CPU path timed on this host before the GPU is used,
+    one process per core (fork pool): trial division on min(8, cores)
+    processes, one triple each (the reference farms one triple per rank,
+    master.c:587-615); Miller-Rabin as GMP mpz_powm on every core; and
+    mpz_probab_prime_p (BPSW) at 10 000 digits beside a full mpz_powm on the
+    same numbers, the ratio carrying it to 50 000 digits."""
+    import multiprocessing as mp
+    global _CPU_PRIMES
+    sys.path.insert(0, str(ROOT / "tests"))
+    from _support import numpy_primes
+    cfg = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())
+    _CPU_PRIMES = numpy_primes(cfg["2"]["prime_limit"])
+    cores = cpu_share()
+    ctx = mp.get_context("fork")
+    out = {}
+    triples = cfg["2"]["triples"]
+    n_td = min(len(triples), cores)
+    t0 = time.perf_counter()
+    with ctx.Pool(n_td) as pool:
+        rows = pool.map(_td_cpu_worker, [(tuple(triples[i]), i, budget_s) for i in range(n_td)])
+    wall = time.perf_counter() - t0
+    rate = sum(K * J * n / dt for n, _, dt in rows)
+    out["td"] = {"value": rate, "unit": "pair*primes/s", "cores": n_td, "kind": "port",
+                 "per_triple_1core_s": float(np.mean([dt * len(_CPU_PRIMES) / n
+                                                      for n, _, dt in rows])),
+                 "sample": (f"{n_td} processes, one triple of the workload each, K=J={K}; each "
+                            f"sieves every {rows[0][1]}th prime of the {len(_CPU_PRIMES)} below "
+                            f"10^8 (small primes included, {rows[0][0]} primes, "
+                            f"{min(dt for _, _, dt in rows):.1f}-{max(dt for _, _, dt in rows):.1f} "
+                            f"s per process); oracle/td_oracle.c restating trial_div_cpu.c, "
+                            f"single-threaded per triple; {wall:.1f} s wall with calibration")}
+    w3 = [tuple(x) + (w,) for x in cfg["3"]["candidates"] for w in (0, 1)]
+    jobs = [(w3[i::cores], 4096, budget_s) for i in range(cores)]
+    bp = [(10000, 5000 + 97 * i, 300 + i, 4001 + 2 * i, 17 + 4 * i, i & 1) for i in range(cores)]
+    with ctx.Pool(cores) as pool:
+        mr_rows = pool.map(_mr_cpu_worker, jobs)
+        bpsw_rows = pool.map(_bpsw_cpu_worker, bp)
+    tph = sum(3600.0 * done / scaled for done, scaled, _, _ in mr_rows)
+    t_test_50k = float(np.mean([scaled / done for done, scaled, _, _ in mr_rows]))
+    ratio = float(np.mean([tb / tp for tb, tp in bpsw_rows]))
+    out["mr"] = {"value": tph, "unit": "tests/hour", "cores": cores, "kind": "port",
+                 "s_per_test_1core": t_test_50k,
+                 "sample": (f"{cores} processes; workload 3 bigints (50 000 digits), GMP "
+                            f"mpz_powm of base 2 over the top 4096 of {mr_rows[0][3]} bits of "
+                            f"d, scaled to the whole exponent (extrapolated); "
+                            f"{sum(r[0] for r in mr_rows)} powms in "
+                            f"{max(r[2] for r in mr_rows):.1f} s per process")}
+    out["mr_reference_verdict"] = {
+        "what": "mpz_probab_prime_p(N, 5) per bigint (BPSW, GMP 6.2), the call of "
+                "miller_rabin.c:91 that worker.c:84 makes",
+        "s_per_bigint_10k_digits": float(np.mean([tb for tb, _ in bpsw_rows])),
+        "bpsw_over_powm_10k_digits": ratio,
+        "s_per_bigint_50k_digits_extrapolated": ratio * t_test_50k,
+        "bigints_per_hour_extrapolated": cores * 3600.0 / (ratio * t_test_50k),
+        "cores": cores,
+        "sample": f"{cores} numbers of 10 000 digits without prime factors below 10^5 (as "
+                  f"Phase-1 survivors), one per process, each timed with mpz_probab_prime_p "
+                  f"and a full base-2 mpz_powm; the ratio times the 50 000-digit powm time "
+                  f"above"}
+    return out
 
 
 def mr_ops_per_squaring(logm: int) -> int:
@@ -99,25 +225,25 @@ def mr_ops_per_squaring(logm: int) -> int:
     return 2 * (4 * T + 2 * Th + 20 * M) + 50 * M
 
 
-def mr_leg(pf, n_bits: int) -> dict:
+def mr_leg(pf, n_bits: int, digits: int = 50000) -> dict:
     """Batched Miller-Rabin at 50 000 digits (workload 3: 128 seeded
     candidates and their decimal reversals, built on the device): time the
     first n_bits squaring steps of a base-2 round over the whole batch with
     HIP events on the context's stream and scale to the full exponent
-    length."""
+    length.  digits != 50000 keeps the workload's b, c, k, j with a = digits."""
     cfg = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())["3"]
-    cands = [tuple(x) + (w,) for x in cfg["candidates"] for w in (0, 1)]
+    cands = [(digits,) + tuple(x[1:]) + (w,) for x in cfg["candidates"] for w in (0, 1)]
     t0 = time.perf_counter()
     ctx = pf.MrContext(candidates=cands)
     setup_s = time.perf_counter() - t0
     ctx.time_squarings(2, 64)                         # warm-up
     ms = ctx.time_squarings(2, n_bits)
-    bits = 166096                                     # squarings per 50 000-digit test
+    bits = int(digits * 3.3219280948873623) - 1       # squarings per test (bits of d)
     step_s = ms / 1e3 / n_bits
     test_s = step_s * bits
     ops = mr_ops_per_squaring(ctx.log_len) * len(cands)
     achieved = ops / step_s / 1e12
-    res = {"workload": "256 bigints (128 N + 128 rev N) at 50 000 digits, base 2",
+    res = {"workload": f"256 bigints (128 N + 128 rev N) at {digits} digits, base 2",
            "limbs": ctx.limbs, "transform_len": 1 << ctx.log_len,
            "us_per_squaring_batch": step_s * 1e6,
            "tests_per_hour_extrapolated": 3600.0 * len(cands) / test_s,
@@ -133,6 +259,88 @@ def mr_leg(pf, n_bits: int) -> dict:
                         "ops_per_squaring_step": ops}}
     ctx.close()
     return res
+
+
+def td_reference_shape(pf, primes: np.ndarray, triple, calls: int = 10) -> dict:
+    """The call the reference's worker makes (worker.c:37-42): trial_div_gpu
+    at its own grid, K = J = 100, on a host prime list (here every prime
+    below 10^8), timed per call end to end and by its gpu_timing_t.  The
+    first call uploads the list; later calls compare the host list with the
+    copy kept from the upload and reuse the device list."""
+    a, b, c = triple
+    t0 = time.perf_counter()
+    rc, _, first = pf.trial_div_gpu(primes, a, b, c, timing=True)
+    first_s = time.perf_counter() - t0
+    assert rc == 0
+    walls, tims = [], []
+    for _ in range(calls):
+        t0 = time.perf_counter()
+        rc, _, tm = pf.trial_div_gpu(primes, a, b, c, timing=True)
+        walls.append(time.perf_counter() - t0)
+        tims.append(tm)
+        assert rc == 0
+    tm = np.mean(np.array(tims), axis=0)
+    wall = float(np.mean(walls))
+    return {"workload": f"trial_div_gpu, K = J = 100, triple ({a},{b},{c}), "
+                        f"{len(primes)} primes below 10^8 (host list)",
+            "first_call_ms": first_s * 1e3,
+            "first_call_timing_ms": {"upload": first[0] * 1e3, "kernel": first[1] * 1e3,
+                                     "download": first[2] * 1e3},
+            "ms_per_call": wall * 1e3,
+            "timing_ms": {"upload": tm[0] * 1e3, "kernel": tm[1] * 1e3, "download": tm[2] * 1e3},
+            "host_ms_per_call": (wall - float(tm.sum())) * 1e3,
+            "pair_primes_per_s": 100 * 100 * len(primes) / wall,
+            "calls": calls}
+
+
+def mr_full_round(pf) -> dict:
+    """Workload 3 as specified: the 256 bigints (128 N + 128 rev N, 50 000
+    digits) built on the device, one base-2 strong-probable-prime round over
+    all of them in one launch sequence, wall-clocked end to end (no
+    extrapolation).  Every bigint of the workload is composite at base 2, so
+    this is also the whole retiring test of bases 2, 3, 5, 7."""
+    cfg = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())["3"]
+    cands = [tuple(x) + (w,) for x in cfg["candidates"] for w in (0, 1)]
+    t0 = time.perf_counter()
+    ctx = pf.MrContext(candidates=cands)
+    build_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    alive, _ = ctx.run([2], residues=False)
+    wall = time.perf_counter() - t0
+    ctx.close()
+    return {"workload": "256 bigints (128 N + 128 rev N) at 50 000 digits, base 2, one launch",
+            "wall_s": wall, "build_setup_s": build_s,
+            "tests_per_hour": 3600.0 * len(cands) / wall,
+            "probable_primes": int(sum(alive))}
+
+
+def mr_step_leg(pf, cands, n_bits: int, total_bits: int, label: str) -> dict:
+    """Seconds per batched squaring step (HIP events on the context's
+    stream) of a base-2 round over `cands`, the kernel's roofline by the
+    transform-count model, and the same for one candidate alone (the plain
+    one-candidate-per-launch path)."""
+    ctx = pf.MrContext(candidates=cands)
+    ctx.time_squarings(2, 64)
+    step_s = ctx.time_squarings(2, n_bits) / 1e3 / n_bits
+    logm = ctx.log_len
+    ops = mr_ops_per_squaring(logm) * len(cands)
+    ctx.close()
+    one = pf.MrContext(candidates=cands[:1])
+    one.time_squarings(2, 64)
+    one_s = one.time_squarings(2, min(n_bits, 1024)) / 1e3 / min(n_bits, 1024)
+    one.close()
+    achieved = ops / step_s / 1e12
+    tph = 3600.0 * len(cands) / (step_s * total_bits)
+    plain = 3600.0 / (one_s * total_bits)
+    return {"workload": label, "transform_len": 1 << logm,
+            "us_per_squaring_batch": step_s * 1e6,
+            "tests_per_hour_extrapolated": tph,
+            "sample": f"{n_bits} of {total_bits} squaring steps timed (HIP events), scaled",
+            "plain_one_candidate": {"us_per_squaring": one_s * 1e6, "tests_per_hour": plain,
+                                    "batched_over_plain": tph / plain},
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": INT32_VALU_PEAK_TOPS,
+                         "unit": "Tops/s", "frac": achieved / INT32_VALU_PEAK_TOPS,
+                         "kernel": f"exp_kernel<{logm}>", "ops_per_squaring_step": ops}}
 
 
 def mr_cpu_baseline(cands, budget_s: float = 15.0, sample_bits: int = 4096) -> dict:
@@ -168,16 +376,16 @@ def workload3(pf, args) -> dict:
     """Workload 3 on its own line: tests/hour of the batched path at 50 000
     digits, extrapolated from the timed squaring steps (every candidate of
     the workload is composite at base 2, so one round per bigint)."""
-    leg = mr_leg(pf, args.mr_bits or 2048)
+    leg = mr_leg(pf, args.mr_bits or 2048, args.digits)
     # the plain path for comparison: one candidate per launch, same kernel
     cfg = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())["3"]
-    one = pf.MrContext(candidates=[tuple(cfg["candidates"][0]) + (0,)])
+    one = pf.MrContext(candidates=[(args.digits,) + tuple(cfg["candidates"][0][1:]) + (0,)])
     one.time_squarings(2, 64)
     one_step_s = one.time_squarings(2, 1024) / 1e3 / 1024
     one.close()
-    plain_tph = 3600.0 / (one_step_s * 166096)
+    plain_tph = 3600.0 / (one_step_s * (int(args.digits * 3.3219280948873623) - 1))
     return {
-        "metric": "Miller-Rabin tests/hour (batched, 50 000 digits)",
+        "metric": f"Miller-Rabin tests/hour (batched, {args.digits} digits)",
         "value": leg["tests_per_hour_extrapolated"], "unit": "tests/hour", "n_gpus": 1,
         "steps": 1, "warmup": 1, "ms_per_step": leg["us_per_squaring_batch"] / 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u32",
@@ -280,7 +488,7 @@ def capacity_estimate(pf, rows, bases_n=1) -> dict:
             "s_per_step": {str(1 << k): v for k, v in per_step.items()}}
 
 
-def workload5(pf, args) -> dict:
+def workload5(pf, args, cap=None) -> dict:
     """Serving: requests of tests/golden/configs.json workload 5 (a uniform in
     [50 000, 100 000] digits, N or rev(N) by the row's coin, bases 2, 3, 5, 7)
     arrive as a seeded Poisson process at `load` times the measured capacity
@@ -289,7 +497,7 @@ def workload5(pf, args) -> dict:
     that arrived in the window (all are drained)."""
     cfg = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())["5"]
     rows = cfg["requests"]
-    cap = capacity_estimate(pf, rows)
+    cap = cap or capacity_estimate(pf, rows)
     rate = args.load * cap["requests_per_s"]
     rng = np.random.default_rng(5)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(rows)))
@@ -297,6 +505,8 @@ def workload5(pf, args) -> dict:
     cands = _mr_cands(rows[:n_req], coin=True)
     _progress(f"workload 5: capacity {cap['requests_per_s']:.3f} req/s, load {args.load}, "
               f"{n_req} requests over {args.serve_seconds} s")
+    if n_req == 0:
+        raise RuntimeError("workload 5: no request arrives in the window")
     s = pf.MrScheduler(cfg["bases"], chunk=args.chunk)
     t0 = time.perf_counter()
     mono0 = time.clock_gettime(time.CLOCK_MONOTONIC)
@@ -347,7 +557,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="seconds of work per process in each CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--mr-bits", type=int, default=2048,
                     help="squaring steps timed per Miller-Rabin sample (0 = skip the MR leg)")
@@ -357,10 +568,15 @@ def main() -> int:
                     help="2: trial-division headline; 3: batched Miller-Rabin at 50 000 digits; "
                          "4: batched Miller-Rabin of mixed sizes; 5: Miller-Rabin serving")
     ap.add_argument("--limit", type=int, default=0, help="workload 4: first N bigints only")
+    ap.add_argument("--digits", type=int, default=50000,
+                    help="workload 3: decimal digits a (the workload's b, c, k, j kept)")
     ap.add_argument("--chunk", type=int, default=256, help="squaring steps per scheduler chunk")
     ap.add_argument("--load", type=float, default=0.70, help="workload 5: arrival rate / capacity")
     ap.add_argument("--serve-seconds", type=float, default=300.0,
                     help="workload 5: length of the arrival window")
+    ap.add_argument("--serve-window", type=float, default=30.0,
+                    help="default line: arrival window of each serving sample at 0.70 and "
+                         "0.95 x capacity (0 = no serving sample)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only; --gpus {args.gpus} is not "
@@ -368,6 +584,11 @@ def main() -> int:
         return 2
 
     pf = _load("pf_amd", ROOT / "prime-finder_amd" / "pf_amd.py")
+    cpu = None
+    if args.workload == 2 and not args.no_cpu_baseline:
+        # before anything touches the GPU: the pool forks this process
+        _progress("CPU baselines (trial division, GMP Miller-Rabin, GMP BPSW)")
+        cpu = cpu_baselines(args.cpu_seconds)
     if args.workload in (3, 4, 5):
         out = {3: workload3, 4: workload4, 5: workload5}[args.workload](pf, args)
         print(json.dumps(out), flush=True)
@@ -460,10 +681,41 @@ def main() -> int:
         "survivors_first_triple": survivors0,
         "device": pf.device_info(),
     }
+    _progress("trial division at the reference's grid (K = J = 100)")
+    out["td_reference_shape"] = td_reference_shape(pf, primes, triples[0])
+    if cpu:
+        out["cpu_baseline"] = cpu["td"]
     if args.mr_bits:
-        out["mr"] = mr_leg(pf, args.mr_bits)
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(primes, triples[0], args.cpu_seconds)
+        cfg3 = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())["3"]
+        w3 = [tuple(x) + (w,) for x in cfg3["candidates"] for w in (0, 1)]
+        w3_100k = [(100000,) + tuple(x[1:]) + (w,) for x in cfg3["candidates"] for w in (0, 1)]
+        _progress("Miller-Rabin: full base-2 round of workload 3 (256 bigints, 50 000 digits)")
+        mr = {"uniform_50k_full_round": mr_full_round(pf)}
+        _progress("Miller-Rabin: squaring-step rates at 50 000 and 100 000 digits")
+        mr["step_50k"] = mr_step_leg(pf, w3, args.mr_bits, 166096,
+                                     "256 bigints (128 N + 128 rev N) at 50 000 digits, base 2")
+        mr["step_100k"] = mr_step_leg(pf, w3_100k, max(256, args.mr_bits // 2), 332192,
+                                      "256 bigints at 100 000 digits (workload 3's b, c, k, j "
+                                      "with a = 100 000; N and rev N), base 2")
+        if args.serve_window > 0:
+            cfg5 = json.loads((ROOT / "tests" / "golden" / "configs.json").read_text())["5"]
+            cap = capacity_estimate(pf, cfg5["requests"])
+            mr["serving"] = {}
+            for load in (0.70, 0.95):
+                _progress(f"Miller-Rabin serving window at {load} x capacity")
+                sargs = argparse.Namespace(**vars(args))
+                sargs.load, sargs.serve_seconds = load, args.serve_window
+                w5 = workload5(pf, sargs, cap)
+                mr["serving"][f"{load:.2f}"] = {
+                    k: w5[k] for k in ("value", "unit", "p50_s", "p99_s", "mean_s", "max_s",
+                                       "arrival_rate_per_s", "ms_per_step")}
+                mr["serving"][f"{load:.2f}"]["requests"] = w5["config"]["requests"]
+                mr["serving"][f"{load:.2f}"]["window_s"] = args.serve_window
+            mr["serving"]["capacity"] = cap
+        if cpu:
+            mr["cpu_baseline"] = cpu["mr"]
+            mr["cpu_reference_verdict"] = cpu["mr_reference_verdict"]
+        out["mr"] = mr
     print(json.dumps(out), flush=True)
     return 0
 

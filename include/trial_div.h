@@ -42,7 +42,8 @@ typedef struct {
  * Synchronous.  Returns 0, or -1 after printing
  * "trial_div_gpu: HIP error ..." to stderr (bits then unchanged).
  * The device copy of pl->primes is cached across calls and refreshed when
- * the pointer, the count or the contents (64-bit checksum) change.
+ * the pointer, the count or the contents change (compared word for word
+ * with a host copy of the last upload).
  * Every prime must satisfy 2 <= p < 2^31. */
 int trial_div_gpu(const prime_list_t *pl,
                   long long a, long long b, long long c,
@@ -91,8 +92,10 @@ int trial_div_launch_kj_ev(const uint32_t *d_primes, uint32_t n_primes,
                            void *ev_lane_start, void *ev_lane_stop);
 
 /* Synchronises the current device, then returns and clears the error latched
- * by earlier launches: 0 = none, 1 = a prime outside [2, 2^31),
- * 2 = more than 65536 list entries below 2^16 (a duplicated list). */
+ * by earlier launches: 0 = none, else a bit mask: 1 = a prime outside
+ * [2, 2^31), 2 = more than 65536 list entries below 2^16, 4 = more than
+ * 1 100 000 entries from 2^16 to the lattice range (2^21) (both: a
+ * duplicated list). */
 int trial_div_check_error(void);
 
 /* trial_div_launch_kj that allocates nothing: no hipMalloc, hipFree, stream
@@ -100,11 +103,12 @@ int trial_div_check_error(void);
  * reference; it is for callers that cannot allocate (code under HIP graph
  * capture, tight serving loops, harnesses that forbid allocation).  Its
  * scratch (small-prime list and setup table, control words, rev(j) decode
- * table, row-kernel work items: 6.97 MB) is one static device array that
+ * table, row-kernel work items, mid-range prime list: 11.37 MB) is one
+ * static device array that
  * each GPU the library runs on holds until the process ends; every kernel
  * runs in order on `stream`.  Limits: K, J in 1..9999 (-1 and a message
- * otherwise) and at most 65536 list entries below 2^16 (latched, reported
- * by trial_div_check_error() as 2).  All calls on one GPU share the array:
+ * otherwise), at most 65536 list entries below 2^16 and 1 100 000 from 2^16
+ * to 2^21 (latched, reported by trial_div_check_error() as 2 and 4).  All calls on one GPU share the array:
  * calls that overlap in time on one GPU (other streams, other host threads)
  * race and must be serialised by the caller.  Same arguments, results and
  * return codes as trial_div_launch_kj. */

@@ -63,7 +63,13 @@ enum {
 };
 
 // ctrl words
-enum { CT_SMALL_COUNT = 0, CT_ERROR = 1, CT_ITEMS = 2, CT_ITEM_OVF = 3, CT_WORDS = 4 };
+enum { CT_SMALL_COUNT = 0, CT_ERROR = 1, CT_ITEMS = 2, CT_ITEM_OVF = 3, CT_MID_COUNT = 4,
+       CT_WORDS = 5 };
+
+// Primes in [TD_SMALL_LIMIT, mid limit) are compacted into a list of at most
+// TD_MID_CAP entries (every prime of [2^16, 2^24) fits) for the split k-loop
+// kernel.
+constexpr uint32_t TD_MID_CAP = 1100000;
 
 // Row-kernel work items: (small prime, kind, part), each at most TD_CH
 // marking steps, so that lanes of one wave get comparable work.
@@ -99,29 +105,48 @@ __device__ uint32_t td_powmod16(uint32_t base, uint64_t e, uint32_t p)
 
 __global__ void td_reset_kernel(uint32_t *ctrl)
 {
-    if (threadIdx.x == 0) ctrl[CT_SMALL_COUNT] = 0u;
+    if (threadIdx.x == 0) {
+        ctrl[CT_SMALL_COUNT] = 0u;
+        ctrl[CT_MID_COUNT] = 0u;
+    }
 }
 
 __global__ __launch_bounds__(TD_BLOCK)
 void td_split_kernel(const uint32_t *__restrict__ primes, uint32_t n,
-                     uint32_t *__restrict__ small, uint32_t *__restrict__ ctrl)
+                     uint32_t *__restrict__ small, uint32_t *__restrict__ mid, uint32_t mid_hi,
+                     uint32_t *__restrict__ ctrl)
 {
     const uint32_t i = blockIdx.x * TD_BLOCK + threadIdx.x;
-    const uint32_t p = i < n ? primes[i] : TD_SMALL_LIMIT;
+    const uint32_t p = i < n ? primes[i] : TD_PMAX - 1u;
     const bool bad = i < n && (p < 2u || p >= TD_PMAX);
     const bool is_small = !bad && p < TD_SMALL_LIMIT;
+    const bool is_mid = !bad && i < n && p >= TD_SMALL_LIMIT && p < mid_hi;
     if (bad) atomicOr(&ctrl[CT_ERROR], 1u);
-    const uint64_t mask = __ballot(is_small);
-    if (mask == 0) return;
     const int lane = __lane_id();
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(&ctrl[CT_SMALL_COUNT], (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    if (is_small) {
-        const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-        if (pos < TD_SMALL_CAP) small[pos] = p;
-        else atomicOr(&ctrl[CT_ERROR], 2u);
+    // wave-ballot compaction: one atomic per wave and list
+    const uint64_t mask = __ballot(is_small);
+    if (mask) {
+        const int leader = __ffsll((unsigned long long)mask) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&ctrl[CT_SMALL_COUNT], (uint32_t)__popcll(mask));
+        base = __shfl(base, leader);
+        if (is_small) {
+            const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+            if (pos < TD_SMALL_CAP) small[pos] = p;
+            else atomicOr(&ctrl[CT_ERROR], 2u);
+        }
+    }
+    const uint64_t mm = __ballot(is_mid);
+    if (mm) {
+        const int leader = __ffsll((unsigned long long)mm) - 1;
+        uint32_t base = 0;
+        if (lane == leader) base = atomicAdd(&ctrl[CT_MID_COUNT], (uint32_t)__popcll(mm));
+        base = __shfl(base, leader);
+        if (is_mid) {
+            const uint32_t pos = base + (uint32_t)__popcll(mm & ((1ull << lane) - 1ull));
+            if (pos < TD_MID_CAP) mid[pos] = p;
+            else atomicOr(&ctrl[CT_ERROR], 4u);
+        }
     }
 }
 
@@ -403,7 +428,7 @@ __device__ __forceinline__ void td_hits(uint32_t k, uint32_t un, uint32_t thrn,
             const uint32_t r = v[l] + 1u;
             uint32_t j;
             if (MERGED) {
-                j = rdec[r];
+                j = td_rdec_arith(r, DJ);    // = rdec[r], without the load's latency
             } else {
                 j = (r % 10u == 0u) ? 0xFFFFFFFFu : td_reverse(r, l + 1);
             }
@@ -678,20 +703,143 @@ void td_large_kernel(const uint32_t *__restrict__ primes, uint32_t n,
     }
 }
 
-// Lattice kernel: one lane per prime >= TD_LAT_MIN (merged j digit groups),
-// hits enumerated directly by td_lattice_prime (td_lattice.h) instead of the
-// K-step loop of td_large_kernel.
+// Mid-range primes (from 2^16 up to the lattice path, merged j digit
+// groups), exact residues: the k loop of td_large_kernel with each prime's
+// rows split over TD_MID_SPLIT lanes (q ranges of k / 10), so that the
+// ~10^5 primes of the range fill the machine (one lane per prime left most
+// SIMDs idle).  The lanes of one prime repeat the setup (they run side by
+// side in one wave, so it costs no extra time) and start their residues at
+// their own first row: u = un0 - k0 rb/rc and the rev residue
+// cm - rev(10 q0) E (one Montgomery product each).  Grid-stride over the
+// compacted list td_split_kernel wrote.
+#ifndef TD_MID_SPLIT_N
+#define TD_MID_SPLIT_N 8
+#endif
+constexpr int TD_MID_SPLIT = TD_MID_SPLIT_N;
+
+template <int DJ>
+__global__ __launch_bounds__(TD_BLOCK)
+void td_mid_kernel(const uint32_t *__restrict__ mid, const uint32_t *__restrict__ ctrl,
+                   long long a, long long b, long long c,
+                   uint32_t K, uint32_t J, const uint16_t *__restrict__ rdec,
+                   uint32_t *__restrict__ bits)
+{
+    const uint32_t count = min(ctrl[CT_MID_COUNT], TD_MID_CAP);
+    const uint32_t Qtot = K / 10u + 1u;           // q = k / 10 runs over [0, Qtot)
+    const int DK = (K >= 1000u) ? 4 : (K >= 100u) ? 3 : (K >= 10u) ? 2 : 1;
+    const uint32_t thr = td_pow10(DJ) - 1u;
+    for (uint32_t g = blockIdx.x * TD_BLOCK + threadIdx.x; g < count * TD_MID_SPLIT;
+         g += gridDim.x * TD_BLOCK) {
+        const uint32_t p = mid[g / TD_MID_SPLIT];
+        const uint32_t part = g % TD_MID_SPLIT;
+        const uint32_t qa = part * Qtot / TD_MID_SPLIT, qb = (part + 1u) * Qtot / TD_MID_SPLIT;
+        if (qa >= qb) continue;
+
+        // ---- per-prime setup (Montgomery, R = 2^32) ----
+        const PfMont m = pf_mont_init(p);
+        const uint32_t ten = pf_mont_in(10u, m);
+        const uint32_t pm = p % 10u;
+        const uint32_t t = pm == 1u ? 9u : pm == 3u ? 3u : pm == 7u ? 7u : 1u;
+        const uint32_t tenInv = pf_mont_in((uint32_t)((1ull + (uint64_t)t * p) / 10u), m);
+        const long long eb1 = a - b - 1;
+        uint32_t pw[3], nw[2];
+        {
+            const uint64_t pe[3] = {(uint64_t)a, (uint64_t)b, (uint64_t)(eb1 >= 0 ? eb1 : 0)};
+            const uint64_t ne[2] = {(uint64_t)c, (uint64_t)(a - c - DJ)};
+            pf_mont_pow_multi<3>(ten, pe, pw, m);
+            pf_mont_pow_multi<2>(tenInv, ne, nw, m);
+        }
+        const uint32_t icS = pf_mont_out(nw[0], m);                                 // 10^-c
+        const uint32_t ra1M = pf_submod(pw[0], m.one, p);                           // ra - 1
+        const uint32_t bstep = pf_mont_mul(pw[1], icS, m);                          // rb / rc
+        const uint32_t bstepM = pf_mont_in(bstep, m);
+        const uint32_t un0 = pf_submod(pf_mont_mul(ra1M, icS, m), 1u, p);           // u at k = 0
+        const uint32_t jinvM = nw[1];                                               // 10^-(a-c-DJ)
+        const uint32_t cm = pf_submod(pf_mont_mul(ra1M, pf_mont_out(jinvM, m), m), 1u, p);
+        uint32_t kresM = pw[2];                                                     // 10^(a-b-lk)
+        const uint32_t thrn = J;
+        for (int lk = 1; lk <= DK; lk++) {
+            const uint32_t klo = td_pow10(lk - 1);
+            const uint32_t khi = min(K, td_pow10(lk) - 1u);
+            const uint32_t qlo = klo / 10u, qhi = khi / 10u;
+            const uint32_t q0 = max(qlo, qa), q1 = min(qhi, qb - 1u);
+            if (q0 <= q1) {
+                const bool kok = (a - b - lk) >= 0 && eb1 >= 0;
+                const uint32_t s0w = td_pow10(lk - 1);
+                const uint32_t eM = pf_mont_mul(kresM, jinvM, m);          // E = kres / X_0
+                const uint32_t th = kok ? thr : 0u;
+                const uint32_t s0 = pf_mont_mul(eM, s0w, m);
+                // rev residue at the lane's first row start k = 10 q0 (the
+                // class's first k, rev = 1, when the lane starts the class)
+                uint32_t w = q0 == qlo ? (lk == 1 ? cm : pf_submod(cm, pf_mont_out(eM, m), p))
+                                       : pf_submod(cm, pf_mont_mul(eM, td_reverse(10u * q0, lk), m), p);
+                // N residue before the lane's first k of the class
+                const uint32_t k0 = max(klo, 10u * q0);
+                uint32_t un = pf_submod(un0, pf_mont_mul(bstepM, k0 - 1u, m), p);
+                int d0 = lk >= 2 ? (int)td_pow10(lk - 2) : 0;
+                int d1 = lk >= 3 ? (int)td_pow10(lk - 3) - 9 * (int)td_pow10(lk - 2) : 0;
+                int d2 = lk >= 4 ? (int)td_pow10(lk - 4) - 9 * ((int)td_pow10(lk - 2) + (int)td_pow10(lk - 3)) : 0;
+                const uint32_t tq0 = pf_mont_mul(eM, d0 >= 0 ? (uint32_t)d0 : p - (uint32_t)(-d0), m);
+                const uint32_t tq1 = pf_mont_mul(eM, d1 >= 0 ? (uint32_t)d1 : p - (uint32_t)(-d1), m);
+                const uint32_t tq2 = pf_mont_mul(eM, d2 >= 0 ? (uint32_t)d2 : p - (uint32_t)(-d2), m);
+                for (uint32_t q = q0; q <= q1; q++) {
+                    const uint32_t kb = q * 10u;
+                    const uint32_t dlo = kb < klo ? klo - kb : 0u;
+                    const uint32_t dhi = min(9u, khi - kb);
+                    uint32_t v[1] = {w};
+                    const uint32_t thv[1] = {th};
+                    for (uint32_t d = 0; d < dlo; d++) v[0] = pf_submod(v[0], s0, p);
+                    for (uint32_t d = dlo; d <= dhi; d++) {
+                        un = pf_submod(un, bstep, p);
+                        const bool h = un < thrn || v[0] < th;
+                        if (__builtin_expect(td_any(h), 0))
+                            if (h) td_hits<DJ, true, 1>(kb + d, un, thrn, v, thv, J, rdec, bits);
+                        v[0] = pf_submod(v[0], s0, p);
+                    }
+                    if (q < q1) {
+                        uint32_t t9 = 0, qq = q;
+                        while (qq % 10u == 9u) { t9++; qq /= 10u; }
+                        w = pf_submod(w, t9 == 0 ? tq0 : t9 == 1 ? tq1 : tq2, p);
+                    }
+                }
+            }
+            kresM = pf_mont_mul(kresM, tenInv, m);
+        }
+    }
+}
+
+// Lattice kernel: primes >= TD_LAT_MIN (merged j digit groups), hits
+// enumerated directly (td_lattice.h) instead of the K-step loop of
+// td_large_kernel.  The searches of one prime differ widely in length (the
+// number of hits and the depth of each continued-fraction descent), so a
+// lane does not keep one prime to the end: each wave first sets up its 64
+// primes together (uniform work; records in LDS), then works through their
+// 4 x 64 condition searches as a queue.  Every iteration runs one descent
+// level on every lane that has a search; a lane whose query is decided
+// marks the hit and starts the next query of its search, and a lane whose
+// search is exhausted takes the next search of the wave's queue (assigned
+// by a ballot prefix count, no atomics), so lanes idle only at the end of
+// the wave's queue.
 #ifndef TD_LAT_MIN_LOG2
-#define TD_LAT_MIN_LOG2 21
+#define TD_LAT_MIN_LOG2 19
 #endif
 constexpr uint32_t TD_LAT_MIN = 1u << TD_LAT_MIN_LOG2;
 static_assert(TD_LAT_MIN >= TD_SMALL_LIMIT, "the lattice path needs p > 10^4 >= J");
+static_assert(TD_LAT_MIN <= (1u << 24), "every prime of [2^16, TD_LAT_MIN) fits the mid list");
 
 struct TdDevMark {
     uint32_t *bits;
     __device__ __forceinline__ void operator()(uint32_t bit) const { td_gmark(bits, bit); }
 };
 
+// LDS record fields per prime (structure of arrays over a wave's 64 primes)
+enum { TL_P = 0, TL_NP, TL_UN1, TL_SIGN, TL_SINVN, TL_CM, TL_SIG, TL_SINV = TL_SIG + 3,
+       TL_FLAGS = TL_SINV + 3, TL_FIELDS };
+
+#ifndef TD_LAT_VARIANT
+#define TD_LAT_VARIANT 0
+#endif
+#if TD_LAT_VARIANT == 1
 template <int DJ>
 __global__ __launch_bounds__(TD_BLOCK)
 void td_lattice_kernel(const uint32_t *__restrict__ primes, uint32_t n,
@@ -704,8 +852,114 @@ void td_lattice_kernel(const uint32_t *__restrict__ primes, uint32_t n,
     const uint32_t p = primes[i];
     if (p < pmin || p >= TD_PMAX) return;
     TdDevMark mark{bits};
-    td_lattice_prime<DJ>(p, a, b, c, K, J, rdec, mark);
+    td_lattice_prime<DJ>(p, a, b, c, K, J, mark);
 }
+#else
+template <int DJ>
+__global__ __launch_bounds__(TD_BLOCK)
+void td_lattice_kernel(const uint32_t *__restrict__ primes, uint32_t n,
+                       long long a, long long b, long long c,
+                       uint32_t K, uint32_t J, const uint16_t *__restrict__ rdec,
+                       uint32_t *__restrict__ bits, uint32_t pmin)
+{
+    __shared__ uint32_t rec[TD_BLOCK / 64][TL_FIELDS][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const uint32_t i = blockIdx.x * TD_BLOCK + threadIdx.x;
+    TdDevMark mark{bits};
+    {   // setup of the wave's primes
+        const uint32_t p = i < n ? primes[i] : 0u;
+        TdLatRec r;
+        r.flags = 0u;
+        if (i < n && p >= pmin && p < TD_PMAX) td_lattice_setup<DJ>(p, a, b, c, K, J, mark, r);
+        uint32_t (*R)[64] = rec[wv];
+        R[TL_P][lane] = r.flags ? r.p : 0u;
+        R[TL_NP][lane] = r.flags ? r.nprime : 0u;
+        R[TL_UN1][lane] = r.flags ? r.un1 : 0u;
+        R[TL_SIGN][lane] = r.flags ? r.sigN : 0u;
+        R[TL_SINVN][lane] = r.flags ? r.sinvN : 0u;
+        R[TL_CM][lane] = r.flags ? r.cm : 0u;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            R[TL_SIG + k][lane] = r.flags ? r.sig[k] : 0u;
+            R[TL_SINV + k][lane] = r.flags ? r.sinv[k] : 0u;
+        }
+        R[TL_FLAGS][lane] = r.flags;
+    }
+    __syncthreads();
+    const uint32_t (*R)[64] = rec[wv];
+    const uint32_t T_rev = td_pow10(DJ) - 1u;
+    constexpr int NI = TD_LAT_CONDS * 64;      // searches of the wave: condition-major
+    // lane state: its search (item, condition, parameters, next t) and query
+    int item = lane, cond = 0;
+    bool have = false;
+    TdLatItem it;
+    PfMont m;
+    uint32_t t0 = 0u;
+    TdFh q;
+    q.live = false;
+    q.imm = false;
+    // load search k into the lane, straight from the LDS records (false
+    // when its condition is off)
+    auto take = [&](int k) -> bool {
+        const int slot = k & 63;
+        cond = k >> 6;
+        if (!((R[TL_FLAGS][slot] >> cond) & 1u)) return false;
+        m.p = R[TL_P][slot];
+        m.nprime = R[TL_NP][slot];
+        m.one = m.r2 = 0u;
+        it.c0 = R[cond == 0 ? TL_UN1 : TL_CM][slot];
+        it.sig = R[cond == 0 ? TL_SIGN : TL_SIG + cond - 1][slot];
+        it.sinv = R[cond == 0 ? TL_SINVN : TL_SINV + cond - 1][slot];
+        it.T = cond == 0 ? J : T_rev;
+        it.W = cond == 0 ? K : td_pow10(cond + 1);
+        t0 = 0u;
+        td_fh_start(q, it.c0, it.sig, it.T, m.p);
+        return true;
+    };
+    have = take(item);
+    int next = 64;                             // wave-uniform: next search to hand out
+    for (;;) {
+#if TD_LAT_VARIANT == 2
+        // levels while most lanes have a live query; the bookkeeping below
+        // then serves every lane that is waiting
+        do {
+            if (have && q.live) td_fh_level(q, it.T);
+        } while (__popcll(__ballot(have && q.live)) >= 48);
+#else
+        if (have && q.live) td_fh_level(q, it.T);
+#endif
+        if (have && !q.live) {                 // decided: mark, then the next query or search
+            uint32_t y;
+            const uint32_t x = td_fh_result(q, it.sinv, m, y);
+            if (x >= it.W - t0) {
+                have = false;
+            } else {
+                const uint32_t t = t0 + x;
+                td_lattice_mark<DJ>(cond, t, y, K, J, mark);
+                t0 = t + 1u;
+                if (t0 >= it.W) {
+                    have = false;
+                } else {
+                    uint32_t cc = y + it.sig;
+                    cc = cc >= m.p ? cc - m.p : cc;
+                    td_fh_start(q, cc, it.sig, it.T, m.p);
+                }
+            }
+        }
+        const uint64_t need = __ballot(!have);
+        if (need && next < NI) {
+            const int rank = __popcll(need & ((1ull << lane) - 1ull));
+            if (!have && next + rank < NI) {
+                item = next + rank;
+                have = take(item);
+            }
+            next += __popcll(need);
+        } else if (!__ballot(have)) {
+            break;
+        }
+    }
+}
+#endif
 
 // ---------------------------------------------------------------------------
 // Host side
@@ -721,6 +975,7 @@ struct TdScratch {
     uint32_t *ctrl = nullptr;    // [CT_WORDS]
     uint16_t *rdec = nullptr;    // [TD_RDEC_SIZE]
     uint32_t *items = nullptr;   // [TD_ITEM_CAP] row-kernel work items
+    uint32_t *mid = nullptr;     // [TD_MID_CAP] compacted mid-range primes
     hipStream_t aux = nullptr;   // row kernel, concurrent with the lane kernel
     hipEvent_t fork = nullptr, join = nullptr;
 };
@@ -734,6 +989,7 @@ void td_scratch_release(TdScratch &s)
     if (s.ctrl) (void)hipFree(s.ctrl);
     if (s.rdec) (void)hipFree(s.rdec);
     if (s.items) (void)hipFree(s.items);
+    if (s.mid) (void)hipFree(s.mid);
     if (s.aux) (void)hipStreamDestroy(s.aux);
     if (s.fork) (void)hipEventDestroy(s.fork);
     if (s.join) (void)hipEventDestroy(s.join);
@@ -753,7 +1009,8 @@ int td_scratch(TdScratch **out)
                   hipMalloc(&s.ctrl, CT_WORDS * sizeof(uint32_t)) == hipSuccess &&
                   hipMemset(s.ctrl, 0, CT_WORDS * sizeof(uint32_t)) == hipSuccess &&
                   hipMalloc(&s.rdec, TD_RDEC_SIZE * sizeof(uint16_t)) == hipSuccess &&
-                  hipMalloc(&s.items, TD_ITEM_CAP * sizeof(uint32_t)) == hipSuccess;
+                  hipMalloc(&s.items, TD_ITEM_CAP * sizeof(uint32_t)) == hipSuccess &&
+                  hipMalloc(&s.mid, TD_MID_CAP * sizeof(uint32_t)) == hipSuccess;
         if (ok) {
             td_rdec_kernel<<<(TD_RDEC_SIZE + 255u) / 256u, 256>>>(s.rdec);
             ok = hipGetLastError() == hipSuccess &&
@@ -803,8 +1060,10 @@ void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long
     // primes >= TD_LAT_MIN enumerate their hits directly when the j digit
     // groups merge; below, and for separate groups, the K-step lane kernel.
     if (n && a - c >= DJ) {
-        td_large_kernel<DJ, true><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
-            d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits, TD_LAT_MIN);
+        const uint32_t mid_blocks = std::min<uint64_t>(
+            2048u, ((uint64_t)n * TD_MID_SPLIT + TD_BLOCK - 1u) / TD_BLOCK);
+        td_mid_kernel<DJ><<<mid_blocks, TD_BLOCK, 0, st>>>(
+            s.mid, s.ctrl, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits);
         td_lattice_kernel<DJ><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
             d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits, TD_LAT_MIN);
     } else if (n) {
@@ -831,7 +1090,8 @@ constexpr size_t TD_NA_TBL   = TD_NA_SMALL + TD_SMALL_CAP;
 constexpr size_t TD_NA_CTRL  = TD_NA_TBL + (size_t)TD_SMALL_CAP * TD_TBL;
 constexpr size_t TD_NA_RDEC  = TD_NA_CTRL + 64;                      // 256-byte aligned
 constexpr size_t TD_NA_ITEMS = TD_NA_RDEC + ((TD_RDEC_SIZE * sizeof(uint16_t) + 255u) / 256u) * 64u;
-constexpr size_t TD_NA_WORDS = TD_NA_ITEMS + TD_ITEM_CAP;           // 6.97 MB
+constexpr size_t TD_NA_MID   = TD_NA_ITEMS + TD_ITEM_CAP;
+constexpr size_t TD_NA_WORDS = TD_NA_MID + TD_MID_CAP;              // 11.37 MB
 static_assert(CT_WORDS <= 64, "control words fit their slot");
 
 static __device__ __align__(256) uint32_t scratch[TD_NA_WORDS];
@@ -869,6 +1129,7 @@ int td_noalloc_scratch(hipStream_t st, TdScratch *out)
     out->ctrl = na.ptr + TD_NA_CTRL;
     out->rdec = (uint16_t *)(na.ptr + TD_NA_RDEC);
     out->items = na.ptr + TD_NA_ITEMS;
+    out->mid = na.ptr + TD_NA_MID;
     return 0;
 }
 
@@ -892,12 +1153,14 @@ int td_launch_with(const uint32_t *d_primes, uint32_t n, long long a, long long 
                    uint32_t K, uint32_t J, uint32_t *d_bits, hipStream_t st,
                    hipEvent_t ev0, hipEvent_t ev1, TdScratch *s)
 {
+    const int dj = J >= 1000u ? 4 : J >= 100u ? 3 : J >= 10u ? 2 : 1;
     td_reset_kernel<<<1, 64, 0, st>>>(s->ctrl);
+    // mid-range primes are listed for the split k loop when the j digit
+    // groups merge (a - c >= D_j); otherwise td_large_kernel takes them all
     if (n)
         td_split_kernel<<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
-            d_primes, n, s->small, s->ctrl);
+            d_primes, n, s->small, s->mid, a - c >= dj ? TD_LAT_MIN : TD_SMALL_LIMIT, s->ctrl);
     td_small_setup_kernel<<<TD_SETUP_GRID, TD_BLOCK, 0, st>>>(s->small, s->ctrl, a, b, c, s->tbl);
-    const int dj = J >= 1000u ? 4 : J >= 100u ? 3 : J >= 10u ? 2 : 1;
     switch (dj) {
     case 1: td_launch_rest<1>(d_primes, n, a, b, c, K, J, d_bits, *s, st, ev0, ev1); break;
     case 2: td_launch_rest<2>(d_primes, n, a, b, c, K, J, d_bits, *s, st, ev0, ev1); break;
@@ -1063,9 +1326,10 @@ int td_run_host(const prime_list_t *pl, long long a, long long b, long long c,
         const int err = td_check_error_code();
         if (err != 0) {
             fprintf(stderr, "%s: %s\n", who,
-                    err == 1 ? "prime outside [2, 2^31) in list"
-                    : err == 2 ? "more than 65536 list entries below 2^16"
-                               : "HIP error (error flag readback)");
+                    (err & 1) ? "prime outside [2, 2^31) in list"
+                    : (err & 2) ? "more than 65536 list entries below 2^16"
+                    : (err & 4) ? "more than 1100000 list entries from 2^16 to the lattice range"
+                                : "HIP error (error flag readback)");
             goto fail;
         }
         PF_TRY(who, hipMemcpy(bits, pc->bits, bb, hipMemcpyDeviceToHost));

@@ -64,66 +64,183 @@ PF_HD uint32_t td_rdec_value(uint32_t R, int d)
     return td_reverse(R, lj);
 }
 
-// Smallest t >= 0 with (c + t sig) mod p < T, and y = that residue;
-// 0xFFFFFFFF when there is none (only for a composite list entry).
-// 1 <= T < p, c < p, 1 <= sig < p; siginvM = sig^-1 in Montgomery form.
+// td_rdec_value for R < 10^4 and d <= 4 without a loop or a table: the
+// reversal of R written with d digits, whose leading zeros (R's trailing
+// zeros) vanish in its value, so it is j itself (R = 1230, d = 4: 0321 =
+// 321 = the reversal of 123).  The digits come from multiply-shift division
+// by 10 (exact below 43 690).  Used on the hit paths, where a table load's
+// latency would stall the whole wave.
+PF_HD uint32_t td_rdec_arith(uint32_t R, int d)
+{
+    const uint32_t q1 = (R * 52429u) >> 19;
+    const uint32_t d0 = R - 10u * q1;
+    const uint32_t q2 = (q1 * 52429u) >> 19;
+    const uint32_t d1 = q1 - 10u * q2;
+    const uint32_t q3 = (q2 * 52429u) >> 19;
+    const uint32_t d2 = q2 - 10u * q3;
+    const uint32_t d3 = q3;
+    const uint32_t v2 = d0 * 10u + d1, v3 = v2 * 10u + d2;
+    return d <= 1 ? d0 : d == 2 ? v2 : d == 3 ? v3 : v3 * 10u + d3;
+}
+
+// ---- first-hit query, in steps ----------------------------------------
 //
-// For c >= T this is the smallest x >= 1 with (sig x) mod p in [L, L + T - 1],
-// L = p - c, solved by a nearest-integer continued-fraction descent: at each
-// level (A, M, [L, R]) with R - L = T - 1, either a multiple of A lands in
-// [L, R] without wrapping (done: w = A ceil(L / A)), or every solution wraps
-// and A x - M y = w in [L, R] turns into ((-M) mod A) y mod A in
+// The smallest t >= 0 with (c + t sig) mod p < T (1 <= T < p, c < p,
+// 1 <= sig < p) and y = that residue.  For c >= T this is the smallest
+// x >= 1 with (sig x) mod p in [L, L + T - 1], L = p - c, found by a
+// nearest-integer continued-fraction descent: at each level (A, M, [L, R])
+// with R - L = T - 1, either a multiple of A lands in [L, R] without
+// wrapping (done: w = A ceil(L / A)), or every solution wraps and
+// A x - M y = w in [L, R] turns into ((-M) mod A) y mod A in
 // [L mod A, R mod A] (the value w is L - (L mod A) plus the next level's);
 // A > M/2 is first replaced by M - A with the interval mirrored (w -> M - w),
-// so A at least halves per level.  The top-level value w is accumulated as
-// +-w_level + tacc (wrapping uint32 arithmetic: the result is in [1, p)),
-// and x = w sig^-1 mod p.
-PF_HD uint32_t td_first_hit(uint32_t c, uint32_t sig, uint32_t T, uint32_t siginvM,
-                            const PfMont &m, uint32_t &y)
+// so A at least halves per level: a prime p < 2^31 needs at most 32 levels.
+// The top-level value w is accumulated as +-w_level + tacc (wrapping uint32
+// arithmetic: the result is in [1, p)), and x = w sig^-1 mod p.  A composite
+// list entry (gcd(sig, p) > 1) can make A divide M or vanish; the search
+// then ends with "no hit" instead of looping.
+struct TdFh {
+    uint32_t c;                 // the query's start value
+    uint32_t L, A, M, tacc, w;
+    int lvl;
+    bool neg, live, fail;       // live: levels still to run
+    bool imm;                   // c < T: t = 0 is the hit
+};
+
+PF_HD void td_fh_start(TdFh &q, uint32_t c, uint32_t sig, uint32_t T, uint32_t p)
 {
-    const uint32_t p = m.p;
-    if (c < T) {
-        y = c;
+    q.c = c;
+    q.L = p - c;
+    q.A = sig;
+    q.M = p;
+    q.tacc = 0u;
+    q.w = 0u;
+    q.lvl = 0;
+    q.neg = false;
+    q.fail = false;
+    q.imm = c < T;
+    q.live = !q.imm;
+    if (q.imm) TD_LAT_COUNT(query);
+}
+
+// One level of the descent; clears q.live when the query is decided.
+PF_HD void td_fh_level(TdFh &q, uint32_t T)
+{
+    TD_LAT_COUNT(level);
+    const bool mir = q.A > q.M - q.A;    // mirror: (M - A) x mod M in [M - R, M - L]
+    const uint32_t R = q.L + T - 1u;
+    const uint32_t A = mir ? q.M - q.A : q.A;
+    const uint32_t L = mir ? q.M - R : q.L;
+    q.tacc += mir ? (q.neg ? 0u - q.M : q.M) : 0u;
+    q.neg = q.neg != mir;
+    const uint32_t Ad = A ? A : 1u;
+    const uint32_t qt = L / Ad;
+    const uint32_t rL = L - qt * Ad;
+    const uint32_t up = rL ? Ad - rL : 0u;       // distance to the next multiple of A
+    const uint32_t mA = q.M % Ad;
+    const bool done = up < T;                    // a multiple lands in [L, R]
+    const bool fail = A == 0u || (!done && mA == 0u) || q.lvl >= 40;
+    if (done || fail) {
+        TD_LAT_COUNT(query);
+        q.live = false;
+        q.fail = fail;
+        q.w = L + up;
+        return;
+    }
+    q.tacc += q.neg ? rL - L : L - rL;
+    q.M = A;
+    q.A = A - mA;
+    q.L = rL;
+    q.lvl++;
+}
+
+// The decided query's t (0xFFFFFFFF: none) and residue y.
+PF_HD uint32_t td_fh_result(const TdFh &q, uint32_t siginvM, const PfMont &m, uint32_t &y)
+{
+    if (q.imm) {
+        y = q.c;
         return 0u;
     }
-    uint32_t L = p - c, A = sig, M = p, tacc = 0u, w = 0u;
-    bool neg = false;
-    // A at least halves per level, so a prime p < 2^31 needs at most 32
-    // levels; a composite list entry (gcd(sig, p) > 1) can make A divide M
-    // or vanish, which ends the search with "no hit" instead of looping.
-    TD_LAT_COUNT(query);
-    for (int lvl = 0;; lvl++) {
-        TD_LAT_COUNT(level);
-        if (A > M - A) {                 // mirror: (M - A) x mod M in [M - R, M - L]
-            const uint32_t R = L + T - 1u;
-            A = M - A;
-            L = M - R;
-            tacc += neg ? 0u - M : M;
-            neg = !neg;
-        }
-        if (lvl >= 40 || A == 0u) { y = 0u; return 0xFFFFFFFFu; }
-        const uint32_t q = L / A;
-        const uint32_t rL = L - q * A;
-        if (rL == 0u) { w = L; break; }
-        if (A - rL < T) { w = L + (A - rL); break; }
-        tacc += neg ? rL - L : L - rL;
-        const uint32_t mA = M % A;
-        if (mA == 0u) { y = 0u; return 0xFFFFFFFFu; }
-        M = A;
-        A -= mA;
-        L = rL;
+    if (q.fail) {
+        y = 0u;
+        return 0xFFFFFFFFu;
     }
-    const uint32_t wt = (neg ? 0u - w : w) + tacc;
-    y = c + wt - p;
+    const uint32_t wt = (q.neg ? 0u - q.w : q.w) + q.tacc;
+    y = q.c + wt - m.p;
     return pf_mont_mul(wt, siginvM, m);
 }
 
-// Every hit of one prime p (odd, 10^4 < p < 2^31) on the K x J grid,
-// a - c >= DJ (D_j = digits of J).  rdec[R] is the j whose rev(j) padded to
-// DJ digits is R; mark(bit) receives bit (k - 1) J + (j - 1).
+// Smallest t >= 0 with (c + t sig) mod p < T, and y = that residue;
+// 0xFFFFFFFF when there is none (only for a composite list entry).
+// siginvM = sig^-1 in Montgomery form.
+PF_HD uint32_t td_first_hit(uint32_t c, uint32_t sig, uint32_t T, uint32_t siginvM,
+                            const PfMont &m, uint32_t &y)
+{
+    TdFh q;
+    td_fh_start(q, c, sig, T, m.p);
+    while (q.live) td_fh_level(q, T);
+    return td_fh_result(q, siginvM, m, y);
+}
+
+// ---- per-prime setup and conditions -----------------------------------
+//
+// Conditions of one prime: cond 0 is N, cond 1..3 the rev classes
+// l_k = 2..4 (the class l_k = 1, 9 values of r, is stepped in the setup).
+constexpr int TD_LAT_CONDS = 4;
+
+// Per-prime constants of the lattice path (the record the kernel keeps in
+// LDS while its lanes work through the conditions of 64 primes).
+struct TdLatRec {
+    uint32_t p, nprime;
+    uint32_t un1, sigN, sinvN;       // N: start value, step, step^-1 (Montgomery)
+    uint32_t cm;                     // rev: start value of every class
+    uint32_t sig[TD_LAT_CONDS - 1];  // rev classes l_k = 2..4: step, step^-1
+    uint32_t sinv[TD_LAT_CONDS - 1];
+    uint32_t flags;                  // bit i: condition i is to be searched
+};
+
+// The search of one condition: hits t in [0, W) with (c0 + t sig) mod p < T.
+struct TdLatItem {
+    uint32_t c0, sig, sinv, T, W;
+};
+
+PF_HD bool td_lattice_item(const TdLatRec &r, int cond, uint32_t K, uint32_t J, uint32_t T_rev,
+                           TdLatItem &it)
+{
+    if (!((r.flags >> cond) & 1u)) return false;
+    if (cond == 0) {
+        it = TdLatItem{r.un1, r.sigN, r.sinvN, J, K};
+    } else {
+        it = TdLatItem{r.cm, r.sig[cond - 1], r.sinv[cond - 1], T_rev, td_pow10(cond + 1)};
+    }
+    return true;
+}
+
+// Mark hit t (residue y) of condition cond when it names a pair of the grid:
+// N: (k, j) = (t + 1, y + 1); rev class l_k: k = t reversed with l_k digits
+// (t ending in 0 names no k of the class), j = the j whose rev(j) padded
+// to DJ digits is y + 1.
 template <int DJ, class Mark>
-PF_HD void td_lattice_prime(uint32_t p, long long a, long long b, long long c, uint32_t K,
-                            uint32_t J, const uint16_t *rdec, Mark &mark)
+PF_HD void td_lattice_mark(int cond, uint32_t t, uint32_t y, uint32_t K, uint32_t J,
+                           Mark &mark)
+{
+    TD_LAT_COUNT(hit);
+    if (cond == 0) {
+        mark(t * J + y);
+        return;
+    }
+    if (t % 10u == 0u) return;
+    const uint32_t k = td_reverse(t, cond + 1);
+    const uint32_t j = td_rdec_arith(y + 1u, DJ);
+    if (k <= K && j <= J) mark((k - 1u) * J + (j - 1u));
+}
+
+// Setup of one prime p (odd, 10^4 < p < 2^31) on the K x J grid, a - c >= DJ
+// (D_j = digits of J): the record r, and the marks of the rev class l_k = 1.
+// mark(bit) receives bit (k - 1) J + (j - 1).
+template <int DJ, class Mark>
+PF_HD void td_lattice_setup(uint32_t p, long long a, long long b, long long c, uint32_t K,
+                            uint32_t J, Mark &mark, TdLatRec &r)
 {
     const PfMont m = pf_mont_init(p);
     const uint32_t ten = pf_mont_in(10u, m);
@@ -149,74 +266,92 @@ PF_HD void td_lattice_prime(uint32_t p, long long a, long long b, long long c, u
     const uint32_t ra1M = pf_submod(raM, m.one, p);                              // 10^a - 1
     const uint32_t bstepM = pf_mont_mul(rbM, icM, m);                            // 10^(b-c)
     const uint32_t bstep = pf_mont_out(bstepM, m);
-    const uint32_t un1 = pf_submod(pf_submod(pf_mont_out(pf_mont_mul(ra1M, icM, m), m), 1u, p),
-                                   bstep, p);                                    // j - 1 at k = 1
     const uint32_t binvM = c >= b ? pw[3] : nw[2];                               // 10^(c-b)
 
-    // ---- N: (k, j) with j - 1 = (un1 - (k - 1) bstep) mod p < J ----
-    {
-        const uint32_t sig = p - bstep;
-        const uint32_t siginvM = p - binvM;                 // (-10^(b-c))^-1
-        uint32_t cc = un1, t0 = 0u;
-        for (;;) {
-            uint32_t y;
-            const uint32_t x = td_first_hit(cc, sig, J, siginvM, m, y);
-            if (x >= K - t0) break;                         // (no overflow: t0 < K)
-            const uint32_t tk = t0 + x;
-            mark(tk * J + y);                               // (k, j) = (tk + 1, y + 1)
-            t0 = tk + 1u;
-            if (t0 >= K) break;
-            cc = y + sig;
-            cc = cc >= p ? cc - p : cc;
-        }
-    }
+    // N: (k, j) with j - 1 = (un1 - (k - 1) bstep) mod p < J
+    r.p = p;
+    r.nprime = m.nprime;
+    r.un1 = pf_submod(pf_submod(pf_mont_out(pf_mont_mul(ra1M, icM, m), m), 1u, p), bstep, p);
+    r.sigN = p - bstep;
+    r.sinvN = p - binvM;                                    // (-10^(b-c))^-1
+    r.flags = 1u;
+    r.cm = 0u;
+    for (int i = 0; i < TD_LAT_CONDS - 1; i++) r.sig[i] = r.sinv[i] = 0u;
 
-    // ---- rev: per digit count lk of k, R - 1 = (cm - rev(k) E_lk) mod p ----
+    // rev: per digit count lk of k, R - 1 = (cm - rev(k) E_lk) mod p
     if (eb1 < 0) return;                                    // no class has a rev term
     const uint32_t jinvM = nw[1];                           // 10^-(a-c-DJ)
     const uint32_t cm = pf_submod(pf_mont_out(pf_mont_mul(ra1M, jinvM, m), m), 1u, p);
+    r.cm = cm;
     uint32_t kresM = pw[2];                                 // 10^(a-b-lk)
     const uint32_t T = td_pow10(DJ) - 1u;
     const int DK = (K >= 1000u) ? 4 : (K >= 100u) ? 3 : (K >= 10u) ? 2 : 1;
     // E_lk = 10^(a-b-lk) / 10^(a-c-DJ); E_lk^-1 = 10^(b-c) 10^(lk-DJ)
     uint32_t einvM = bstepM;
     for (int d = 1; d < DJ; d++) einvM = pf_mont_mul(einvM, tenInv, m);
-    for (int lk = 1; lk <= DK; lk++) {
-        if (a - b - lk < 0) break;
+    // (unrolled: the record's arrays are indexed by constants, so they stay
+    // in registers)
+#if defined(__HIPCC__)
+#pragma unroll
+#endif
+    for (int lk = 1; lk <= TD_LAT_CONDS; lk++) {
+        if (lk > DK || a - b - lk < 0) break;
         const uint32_t E = pf_mont_out(pf_mont_mul(kresM, jinvM, m), m);
-        const uint32_t W = td_pow10(lk);
         if (lk == 1) {
             uint32_t v = cm;
-            for (uint32_t r = 1u; r < W; r++) {
+            for (uint32_t rr = 1u; rr < 10u; rr++) {
                 TD_LAT_COUNT(step);
                 v = pf_submod(v, E, p);
-                if (v < T && r % 10u != 0u) {
-                    const uint32_t k = td_reverse(r, lk);
-                    const uint32_t j = rdec[v + 1u];
-                    if (k <= K && j <= J) mark((k - 1u) * J + (j - 1u));
+                if (v < T) {
+                    const uint32_t j = td_rdec_arith(v + 1u, DJ);
+                    if (rr <= K && j <= J) mark((rr - 1u) * J + (j - 1u));
                 }
             }
         } else {
-            const uint32_t sig = p - E;
-            const uint32_t siginvM = p - einvM;
-            uint32_t cc = cm, t0 = 0u;
-            for (;;) {
-                uint32_t y;
-                const uint32_t x = td_first_hit(cc, sig, T, siginvM, m, y);
-                if (x >= W - t0) break;                     // (no overflow: t0 < W)
-                const uint32_t r = t0 + x;
-                if (r % 10u != 0u) {
-                    const uint32_t k = td_reverse(r, lk);
-                    const uint32_t j = rdec[y + 1u];
-                    if (k <= K && j <= J) mark((k - 1u) * J + (j - 1u));
-                }
-                t0 = r + 1u;
-                if (t0 >= W) break;
-                cc = y + sig;
-                cc = cc >= p ? cc - p : cc;
-            }
+            r.sig[lk - 2] = p - E;
+            r.sinv[lk - 2] = p - einvM;
+            r.flags |= 1u << (lk - 1);
         }
         kresM = pf_mont_mul(kresM, tenInv, m);
         einvM = pf_mont_mul(einvM, ten, m);
+    }
+}
+
+// Every hit of one condition, in order of t: a first-hit query from t0,
+// the hit marked, the next query from t + 1.
+template <int DJ, class Mark>
+PF_HD void td_lattice_run(const TdLatRec &r, int cond, const TdLatItem &it, uint32_t K, uint32_t J,
+                          Mark &mark)
+{
+    PfMont m;
+    m.p = r.p;
+    m.nprime = r.nprime;
+    m.one = m.r2 = 0u;                  // (not used by the queries)
+    uint32_t cc = it.c0, t0 = 0u;
+    for (;;) {
+        uint32_t y;
+        const uint32_t x = td_first_hit(cc, it.sig, it.T, it.sinv, m, y);
+        if (x >= it.W - t0) break;                          // (no overflow: t0 < W)
+        const uint32_t t = t0 + x;
+        td_lattice_mark<DJ>(cond, t, y, K, J, mark);
+        t0 = t + 1u;
+        if (t0 >= it.W) break;
+        cc = y + it.sig;
+        cc = cc >= r.p ? cc - r.p : cc;
+    }
+}
+
+// Every hit of one prime, condition by condition (the order of the
+// reference model; the kernel interleaves the same searches across lanes).
+template <int DJ, class Mark>
+PF_HD void td_lattice_prime(uint32_t p, long long a, long long b, long long c, uint32_t K,
+                            uint32_t J, Mark &mark)
+{
+    TdLatRec r;
+    td_lattice_setup<DJ>(p, a, b, c, K, J, mark, r);
+    for (int cond = 0; cond < TD_LAT_CONDS; cond++) {
+        TdLatItem it;
+        if (td_lattice_item(r, cond, K, J, td_pow10(DJ) - 1u, it))
+            td_lattice_run<DJ>(r, cond, it, K, J, mark);
     }
 }

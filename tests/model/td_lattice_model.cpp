@@ -18,17 +18,15 @@ extern "C" int lattice_sieve(const uint32_t *primes, uint32_t n, long long a, lo
 {
     const int DJ = J >= 1000u ? 4 : J >= 100u ? 3 : J >= 10u ? 2 : 1;
     if (K < 1u || J < 1u || K > 9999u || J > 9999u || a - c < DJ) return -1;
-    static uint16_t rdec[10000];
-    for (uint32_t R = 0; R < td_pow10(DJ); R++) rdec[R] = (uint16_t)td_rdec_value(R, DJ);
     HostMark mark{bits};
     for (uint32_t i = 0; i < n; i++) {
         const uint32_t p = primes[i];
         if (p <= 9999u || p >= 0x80000000u || (p & 1u) == 0u) continue;
         switch (DJ) {
-        case 1: td_lattice_prime<1>(p, a, b, c, K, J, rdec, mark); break;
-        case 2: td_lattice_prime<2>(p, a, b, c, K, J, rdec, mark); break;
-        case 3: td_lattice_prime<3>(p, a, b, c, K, J, rdec, mark); break;
-        default: td_lattice_prime<4>(p, a, b, c, K, J, rdec, mark); break;
+        case 1: td_lattice_prime<1>(p, a, b, c, K, J, mark); break;
+        case 2: td_lattice_prime<2>(p, a, b, c, K, J, mark); break;
+        case 3: td_lattice_prime<3>(p, a, b, c, K, J, mark); break;
+        default: td_lattice_prime<4>(p, a, b, c, K, J, mark); break;
         }
     }
     return 0;
