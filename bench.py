@@ -8,10 +8,14 @@ every prime.  Primes and bitmaps are resident in HBM before the timed region.
 
 Prints one JSON line:  value = pair*primes/s over the whole step
 (8 * 4096^2 * 5 761 455 / seconds per step).  "roofline" rates the dominant
-kernel (the per-prime lane kernel) from its own HIP-event durations on the
-stream it runs on; "cpu_baseline" times the CPU restatement of the
-reference's trial_div_cpu.c on one core, on a bounded sample of the same
-workload.
+kernel (the lattice search of the primes from trial_div_lattice_min() up;
+the row kernel and the split k loop of the primes below run beside it on a
+second stream) from its own HIP-event durations on the stream it runs on,
+against its algorithmic INT32 ops (units counted on the host, ops per unit
+in TD_OPS_PER_UNIT);
+"cpu_baseline" times the CPU restatement of the reference's trial_div_cpu.c
+on up to eight processes, one triple each, on a bounded sample of the
+workload's primes.
 """
 from __future__ import annotations
 
@@ -29,19 +33,23 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 K = J = 4096
 DJ = 4
-# Algorithmic INT32 VALU ops of the lane kernel per (prime, k): two residues
-# (N, and rev(N) with its j digit groups merged), each advanced and tested
-# once per k.  Primes below FIX_MIN keep exact residues: a 3-op modular
-# subtraction and 1 compare each (8 ops); primes from FIX_MIN up track
-# fixed-point images round(x 2^32 / p), where the subtraction wraps by
-# itself: 1 subtraction and 1 compare each (4 ops; the rare candidates are
-# recomputed exactly on the side).  Valid while a - c >= D_j, which every
-# triple of the workload satisfies (c = 100, a >= 50000).
-OPS_PER_PRIME_K_EXACT = 4 * 2
-OPS_PER_PRIME_K_FIXED = 2 * 2
-FIX_MIN = 1 << 24
+# Algorithmic INT32 ops per unit of work of the per-prime kernels for primes
+# >= 2^16 (the units are counted exactly on the host by trial_div_work, with
+# the lattice kernel's own per-prime routine):
+#   modmul  32-bit Montgomery multiplication: 4 multiply-class + 4 add-class;
+#   level   one continued-fraction level of a first-hit search: two unsigned
+#           divisions by the same divisor (10 ops each: reciprocal estimate,
+#           multiply-back, correct) and 25 compare/select/add ops;
+#   query   start and finish of a first-hit search (residue, bound check,
+#           next start): 10;
+#   hit     k and j decoded from a hit and the bit marked: 12;
+#   step    one direct step of the one-digit rev class: modular subtraction
+#           and compare, 4;
+#   kstep   one k of the split k loop (primes from 2^16 below the lattice
+#           range, beside the lattice kernel on the second stream): two
+#           residues advanced and compared, 8.
+TD_OPS_PER_UNIT = {"modmul": 8, "level": 45, "query": 10, "hit": 12, "step": 4, "kstep": 8}
 INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12      # 39.32 Tops/s (256 CUs, wave64, 2.4 GHz)
-SMALL_LIMIT = 1 << 16                               # primes below go to the LDS row kernel
 # HBM bytes per launch of the lane kernel on this workload, measured with
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 256 MiB copy kernel's
 # readings: x2.0 and x1.0): 23.3 MB read (the prime list) + 535.5 MB written
@@ -599,9 +607,12 @@ def main() -> int:
     triples = cfg["triples"]
     primes = pf.primes_below(cfg["prime_limit"])
     n = len(primes)
-    n_large = int((primes >= SMALL_LIMIT).sum())
-    n_fixed = int((primes >= FIX_MIN).sum())
-    lane_ops = K * (OPS_PER_PRIME_K_EXACT * (n_large - n_fixed) + OPS_PER_PRIME_K_FIXED * n_fixed)
+    _progress("algorithmic work of the per-prime kernels, counted on the host")
+    works = [pf.trial_div_work(primes, a, b, c, K, J, cpu_share()) for (a, b, c) in triples]
+    work_avg = {f: float(np.mean([w[f] for w in works])) for f in pf.TD_WORK_FIELDS}
+    # the events bracket the lattice kernel; the k loop below it runs beside
+    # it on the second stream, with the row kernel
+    alg_ops = sum(TD_OPS_PER_UNIT[f] * work_avg[f] for f in TD_OPS_PER_UNIT if f != "kstep")
     nw = (K * J + 31) // 32
 
     d_primes = pf.DeviceBuffer.from_array(primes)
@@ -648,7 +659,7 @@ def main() -> int:
     ms_per_step = wall / args.steps * 1e3
     work = len(triples) * K * J * n
     value = work / (wall / args.steps)
-    achieved = lane_ops / avg_lane_s / 1e12
+    achieved = alg_ops / avg_lane_s / 1e12
 
     bits0 = d_bits[0].download()
     survivors0 = K * J - int(np.unpackbits(bits0.view(np.uint8)).sum())
@@ -675,9 +686,12 @@ def main() -> int:
                      "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE); "
                                      "algorithmic 25.1e6 (prime list + bitmap), the rest are "
                                      "hit atomics; 0.11 TB/s, far below the 8 TB/s roof",
-                     "kernel": "td_large_kernel<4,merged>",
+                     "kernel": "td_lattice_kernel<4>",
                      "kernel_avg_ms": avg_lane_s * 1e3,
-                     "ops_per_launch": lane_ops},
+                     "ops_per_launch": alg_ops,
+                     "work_per_launch": work_avg,
+                     "ops_per_unit": TD_OPS_PER_UNIT,
+                     "lattice_min": int(L.trial_div_lattice_min())},
         "survivors_first_triple": survivors0,
         "device": pf.device_info(),
     }

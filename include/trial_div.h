@@ -81,8 +81,11 @@ int trial_div_launch_kj(const uint32_t *d_primes, uint32_t n_primes,
                         uint32_t *d_bits, void *stream);
 
 /* trial_div_launch_kj that also records two hipEvent_t on `stream` around
- * its dominant kernel (the per-prime lane kernel for primes >= 2^16), so a
- * benchmark can time that kernel on the stream it runs on.  Either event may
+ * its dominant kernel (the lattice search of the primes from
+ * trial_div_lattice_min() up, or the k-loop kernel of every prime from 2^16
+ * when the j digit groups do not merge), so a benchmark can time it on the
+ * stream it runs on; the kernels of the smaller primes run beside it on a
+ * second stream.  Either event may
  * be NULL.  No counterpart in the reference (its bench times the whole
  * launch, tests/bench_gpu.cu:53-62). */
 int trial_div_launch_kj_ev(const uint32_t *d_primes, uint32_t n_primes,
@@ -94,7 +97,7 @@ int trial_div_launch_kj_ev(const uint32_t *d_primes, uint32_t n_primes,
 /* Synchronises the current device, then returns and clears the error latched
  * by earlier launches: 0 = none, else a bit mask: 1 = a prime outside
  * [2, 2^31), 2 = more than 65536 list entries below 2^16, 4 = more than
- * 1 100 000 entries from 2^16 to the lattice range (2^21) (both: a
+ * 1 100 000 entries from 2^16 to the lattice range (2^18) (both: a
  * duplicated list). */
 int trial_div_check_error(void);
 
@@ -108,7 +111,8 @@ int trial_div_check_error(void);
  * each GPU the library runs on holds until the process ends; every kernel
  * runs in order on `stream`.  Limits: K, J in 1..9999 (-1 and a message
  * otherwise), at most 65536 list entries below 2^16 and 1 100 000 from 2^16
- * to 2^21 (latched, reported by trial_div_check_error() as 2 and 4).  All calls on one GPU share the array:
+ * to 2^18 (latched, reported by trial_div_check_error() as 2 and 4).  All
+ * calls on one GPU share the array:
  * calls that overlap in time on one GPU (other streams, other host threads)
  * race and must be serialised by the caller.  Same arguments, results and
  * return codes as trial_div_launch_kj. */
@@ -116,6 +120,31 @@ int trial_div_launch_kj_noalloc(const uint32_t *d_primes, uint32_t n_primes,
                                 long long a, long long b, long long c,
                                 uint32_t K, uint32_t J,
                                 uint32_t *d_bits, void *stream);
+
+/* The lower bound of the primes whose hits the lattice kernel enumerates
+ * directly (when the j digit groups merge, a - c >= D_j); primes from 2^16
+ * below it take the split k loop.  No counterpart in the reference. */
+uint32_t trial_div_lattice_min(void);
+
+/* Algorithmic work of one trial_div_launch_kj call, counted on the host
+ * (no GPU call) with `threads` threads, for rating the kernels: out[] is
+ * indexed by the TD_WORK_* fields below.  The lattice counts come from the
+ * lattice kernel's own per-prime routine run with counting hooks.  Returns 0,
+ * or -1 for invalid arguments.  No counterpart in the reference. */
+enum {
+    TD_WORK_PRIME = 0,   /* primes on the lattice path                      */
+    TD_WORK_MODMUL,      /* their 32-bit Montgomery multiplications         */
+    TD_WORK_QUERY,       /* first-hit queries                               */
+    TD_WORK_LEVEL,       /* continued-fraction levels of those queries      */
+    TD_WORK_HIT,         /* hits found (before the k <= K, j <= J filter)   */
+    TD_WORK_STEP,        /* direct steps of the one-digit rev class         */
+    TD_WORK_KSTEP,       /* prime x k steps of the k-loop kernels (>= 2^16) */
+    TD_WORK_FIELDS
+};
+int trial_div_work(const uint32_t *primes, uint32_t n_primes,
+                   long long a, long long b, long long c,
+                   uint32_t K, uint32_t J, uint32_t threads,
+                   uint64_t out[TD_WORK_FIELDS]);
 
 /* Survivor extraction of worker.c:45-54 on the device: the pairs (k, j) of
  * a K x J bitmap (device pointer, layout as trial_div_gpu_kj) whose bit is

@@ -1,7 +1,8 @@
 """Build recipe for the gfx950 library ``lib/libpfamd.so``.
 
 Every ``csrc/*.hip`` file is compiled by hipcc for gfx950 only, every
-``csrc/*.c`` file by gcc, and the objects are linked into one shared library
+``csrc/*.c`` file by gcc, every ``csrc/*.cpp`` file (host-only code) by g++,
+and the objects are linked into one shared library
 whose exported C symbols are declared in ``include/*.h``.  The compiled
 callers of ``callers/`` (``worker_dropin``, ``bench_gpu_dropin``,
 ``mr_cpp_check``: the reference's worker, bench and runner call sites) are
@@ -33,6 +34,8 @@ HIP_FLAGS = [
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]
 C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-Wall", "-Wextra", f"-I{ROOT / 'include'}"]
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-pthread", "-Wall", "-Wextra",
+             f"-I{ROOT / 'include'}", f"-I{CSRC}"]
 
 
 def _run(cmd: list[str]) -> None:
@@ -56,6 +59,9 @@ def build(verbose: bool = False) -> Path:
     for src in sorted(CSRC.glob("*.c")):
         obj = OBJ / (src.stem + ".c.o")
         jobs.append(["gcc", *C_FLAGS, "-c", str(src), "-o", str(obj)])
+    for src in sorted(CSRC.glob("*.cpp")):
+        obj = OBJ / (src.stem + ".cpp.o")
+        jobs.append(["g++", *CXX_FLAGS, "-c", str(src), "-o", str(obj)])
     if verbose:
         for j in jobs:
             print(" ".join(j))
@@ -63,8 +69,9 @@ def build(verbose: bool = False) -> Path:
         list(ex.map(_run, jobs))
     objs = [str(OBJ / (s.stem + ".o")) for s in sorted(CSRC.glob("*.hip"))]
     objs += [str(OBJ / (s.stem + ".c.o")) for s in sorted(CSRC.glob("*.c"))]
+    objs += [str(OBJ / (s.stem + ".cpp.o")) for s in sorted(CSRC.glob("*.cpp"))]
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *objs])
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", str(tmp), *objs, "-pthread"])
     shutil.move(str(tmp), str(LIB))
     # compiled callers of the drop-in boundary (callers/): the reference's
     # worker (C), its tests/bench_gpu.cu (C++, trial_div_launch) and its

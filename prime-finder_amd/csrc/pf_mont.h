@@ -9,6 +9,12 @@
 #define PF_HD inline
 #endif
 
+// Work counter for host-side models (a no-op in the library): one call per
+// Montgomery multiplication.
+#ifndef PF_MONT_COUNT
+#define PF_MONT_COUNT() ((void)0)
+#endif
+
 // High 32 bits of a 32 x 32-bit product (v_mul_hi_u32 on the device).
 PF_HD uint32_t pf_mulhi(uint32_t a, uint32_t b)
 {
@@ -50,6 +56,7 @@ PF_HD PfMont pf_mont_init(uint32_t p)
 // a * b * R^{-1} mod p, for a, b < p.
 PF_HD uint32_t pf_mont_mul(uint32_t a, uint32_t b, const PfMont &m)
 {
+    PF_MONT_COUNT();
     const uint64_t t = (uint64_t)a * b;
     const uint32_t lo = (uint32_t)t;
     const uint32_t hi = (uint32_t)(t >> 32);

@@ -25,12 +25,19 @@
 //                          owns a band of TD_ROWS rows in LDS, lanes spread
 //                          over (row, prime), marks are LDS atomics and the
 //                          band is OR-ed into HBM once;
-//   td_large_kernel        one lane per prime >= 2^16: Montgomery setup, then
-//                          a k loop of 3-op modular steps with a rare
-//                          divergent hit path (global atomic OR).  The rev
-//                          condition's j digit groups share one residue when
-//                          a-c >= D_j (see td_hits), so a k step costs two
-//                          residue updates and two compares.
+//   td_large_kernel        one lane per prime >= 2^16 when the rev
+//                          condition's j digit groups do not merge (a-c < D_j):
+//                          Montgomery setup, then a k loop of 3-op modular
+//                          steps with a rare divergent hit path (global
+//                          atomic OR);
+//   td_mid_kernel          primes in [2^16, TD_LAT_MIN) with merged groups (one
+//                          rev residue per k, see td_hits): the same k loop,
+//                          TD_MID_SPLIT lanes per prime over slices of k;
+//   td_lattice_kernel      primes >= TD_LAT_MIN with merged groups: each
+//                          condition is one residue sequence c0 + t sig mod p,
+//                          and its hits are found one by one by a
+//                          continued-fraction first-hit search (td_lattice.h)
+//                          instead of stepping every k.
 #include "pf_hip.h"
 #include "td_lattice.h"
 #include "../../include/trial_div.h"
@@ -810,18 +817,12 @@ void td_mid_kernel(const uint32_t *__restrict__ mid, const uint32_t *__restrict_
 
 // Lattice kernel: primes >= TD_LAT_MIN (merged j digit groups), hits
 // enumerated directly (td_lattice.h) instead of the K-step loop of
-// td_large_kernel.  The searches of one prime differ widely in length (the
-// number of hits and the depth of each continued-fraction descent), so a
-// lane does not keep one prime to the end: each wave first sets up its 64
-// primes together (uniform work; records in LDS), then works through their
-// 4 x 64 condition searches as a queue.  Every iteration runs one descent
-// level on every lane that has a search; a lane whose query is decided
-// marks the hit and starts the next query of its search, and a lane whose
-// search is exhausted takes the next search of the wave's queue (assigned
-// by a ballot prefix count, no atomics), so lanes idle only at the end of
-// the wave's queue.
+// td_large_kernel: one lane per prime, its four condition searches in turn.
+// (A per-wave queue that hands exhausted lanes the next search balanced the
+// lanes but cost more in bookkeeping than it saved: 2.29 against 1.92 ms
+// per 4096 x 4096 triple over the primes below 10^8.)
 #ifndef TD_LAT_MIN_LOG2
-#define TD_LAT_MIN_LOG2 19
+#define TD_LAT_MIN_LOG2 18
 #endif
 constexpr uint32_t TD_LAT_MIN = 1u << TD_LAT_MIN_LOG2;
 static_assert(TD_LAT_MIN >= TD_SMALL_LIMIT, "the lattice path needs p > 10^4 >= J");
@@ -832,20 +833,11 @@ struct TdDevMark {
     __device__ __forceinline__ void operator()(uint32_t bit) const { td_gmark(bits, bit); }
 };
 
-// LDS record fields per prime (structure of arrays over a wave's 64 primes)
-enum { TL_P = 0, TL_NP, TL_UN1, TL_SIGN, TL_SINVN, TL_CM, TL_SIG, TL_SINV = TL_SIG + 3,
-       TL_FLAGS = TL_SINV + 3, TL_FIELDS };
-
-#ifndef TD_LAT_VARIANT
-#define TD_LAT_VARIANT 0
-#endif
-#if TD_LAT_VARIANT == 1
 template <int DJ>
 __global__ __launch_bounds__(TD_BLOCK)
 void td_lattice_kernel(const uint32_t *__restrict__ primes, uint32_t n,
                        long long a, long long b, long long c,
-                       uint32_t K, uint32_t J, const uint16_t *__restrict__ rdec,
-                       uint32_t *__restrict__ bits, uint32_t pmin)
+                       uint32_t K, uint32_t J, uint32_t *__restrict__ bits, uint32_t pmin)
 {
     const uint32_t i = blockIdx.x * TD_BLOCK + threadIdx.x;
     if (i >= n) return;
@@ -854,112 +846,6 @@ void td_lattice_kernel(const uint32_t *__restrict__ primes, uint32_t n,
     TdDevMark mark{bits};
     td_lattice_prime<DJ>(p, a, b, c, K, J, mark);
 }
-#else
-template <int DJ>
-__global__ __launch_bounds__(TD_BLOCK)
-void td_lattice_kernel(const uint32_t *__restrict__ primes, uint32_t n,
-                       long long a, long long b, long long c,
-                       uint32_t K, uint32_t J, const uint16_t *__restrict__ rdec,
-                       uint32_t *__restrict__ bits, uint32_t pmin)
-{
-    __shared__ uint32_t rec[TD_BLOCK / 64][TL_FIELDS][64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const uint32_t i = blockIdx.x * TD_BLOCK + threadIdx.x;
-    TdDevMark mark{bits};
-    {   // setup of the wave's primes
-        const uint32_t p = i < n ? primes[i] : 0u;
-        TdLatRec r;
-        r.flags = 0u;
-        if (i < n && p >= pmin && p < TD_PMAX) td_lattice_setup<DJ>(p, a, b, c, K, J, mark, r);
-        uint32_t (*R)[64] = rec[wv];
-        R[TL_P][lane] = r.flags ? r.p : 0u;
-        R[TL_NP][lane] = r.flags ? r.nprime : 0u;
-        R[TL_UN1][lane] = r.flags ? r.un1 : 0u;
-        R[TL_SIGN][lane] = r.flags ? r.sigN : 0u;
-        R[TL_SINVN][lane] = r.flags ? r.sinvN : 0u;
-        R[TL_CM][lane] = r.flags ? r.cm : 0u;
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            R[TL_SIG + k][lane] = r.flags ? r.sig[k] : 0u;
-            R[TL_SINV + k][lane] = r.flags ? r.sinv[k] : 0u;
-        }
-        R[TL_FLAGS][lane] = r.flags;
-    }
-    __syncthreads();
-    const uint32_t (*R)[64] = rec[wv];
-    const uint32_t T_rev = td_pow10(DJ) - 1u;
-    constexpr int NI = TD_LAT_CONDS * 64;      // searches of the wave: condition-major
-    // lane state: its search (item, condition, parameters, next t) and query
-    int item = lane, cond = 0;
-    bool have = false;
-    TdLatItem it;
-    PfMont m;
-    uint32_t t0 = 0u;
-    TdFh q;
-    q.live = false;
-    q.imm = false;
-    // load search k into the lane, straight from the LDS records (false
-    // when its condition is off)
-    auto take = [&](int k) -> bool {
-        const int slot = k & 63;
-        cond = k >> 6;
-        if (!((R[TL_FLAGS][slot] >> cond) & 1u)) return false;
-        m.p = R[TL_P][slot];
-        m.nprime = R[TL_NP][slot];
-        m.one = m.r2 = 0u;
-        it.c0 = R[cond == 0 ? TL_UN1 : TL_CM][slot];
-        it.sig = R[cond == 0 ? TL_SIGN : TL_SIG + cond - 1][slot];
-        it.sinv = R[cond == 0 ? TL_SINVN : TL_SINV + cond - 1][slot];
-        it.T = cond == 0 ? J : T_rev;
-        it.W = cond == 0 ? K : td_pow10(cond + 1);
-        t0 = 0u;
-        td_fh_start(q, it.c0, it.sig, it.T, m.p);
-        return true;
-    };
-    have = take(item);
-    int next = 64;                             // wave-uniform: next search to hand out
-    for (;;) {
-#if TD_LAT_VARIANT == 2
-        // levels while most lanes have a live query; the bookkeeping below
-        // then serves every lane that is waiting
-        do {
-            if (have && q.live) td_fh_level(q, it.T);
-        } while (__popcll(__ballot(have && q.live)) >= 48);
-#else
-        if (have && q.live) td_fh_level(q, it.T);
-#endif
-        if (have && !q.live) {                 // decided: mark, then the next query or search
-            uint32_t y;
-            const uint32_t x = td_fh_result(q, it.sinv, m, y);
-            if (x >= it.W - t0) {
-                have = false;
-            } else {
-                const uint32_t t = t0 + x;
-                td_lattice_mark<DJ>(cond, t, y, K, J, mark);
-                t0 = t + 1u;
-                if (t0 >= it.W) {
-                    have = false;
-                } else {
-                    uint32_t cc = y + it.sig;
-                    cc = cc >= m.p ? cc - m.p : cc;
-                    td_fh_start(q, cc, it.sig, it.T, m.p);
-                }
-            }
-        }
-        const uint64_t need = __ballot(!have);
-        if (need && next < NI) {
-            const int rank = __popcll(need & ((1ull << lane) - 1ull));
-            if (!have && next + rank < NI) {
-                item = next + rank;
-                have = take(item);
-            }
-            next += __popcll(need);
-        } else if (!__ballot(have)) {
-            break;
-        }
-    }
-}
-#endif
 
 // ---------------------------------------------------------------------------
 // Host side
@@ -1036,12 +922,14 @@ void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long
 {
     const uint32_t row_groups = (K + TD_ROWS - 1u) / TD_ROWS;
     const size_t lds = ((size_t)TD_ROWS * J / 32u + 2u) * sizeof(uint32_t);
-    // The row kernel (latency-bound) runs on the auxiliary stream beside the
-    // lane kernel (VALU-bound); both only OR into the bitmap.  The join
-    // before returning keeps the shared scratch (table, items) from being
-    // overwritten by the next call while the row kernel still reads it.
-    // Without an auxiliary stream (the allocation-free launch) both kernels
-    // run in order on st.
+    // The row kernel and the mid-range k loop (both latency-bound: few waves,
+    // long dependent chains, frequent hit paths) run in turn on the
+    // auxiliary stream beside the lattice or k-loop kernel of the large
+    // primes (VALU-bound); all of them only OR into the bitmap.  The join
+    // before returning keeps the shared scratch (table, items, mid list)
+    // from being overwritten by the next call while they still read it.
+    // Without an auxiliary stream (the allocation-free launch) every kernel
+    // runs in order on st.
     const hipStream_t rs = s.aux ? s.aux : st;
     if (a - c >= DJ) td_items_kernel<DJ><<<1, 1024, 0, st>>>(s.tbl, s.ctrl, J, s.items);
     if (s.aux) {
@@ -1053,19 +941,22 @@ void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long
             s.tbl, s.ctrl, s.items, s.rdec + TD_RDEC_OFF[DJ], K, J, d_bits);
     else
         td_rows_kernel<DJ><<<row_groups, TD_BLOCK, lds, rs>>>(s.tbl, s.ctrl, K, J, d_bits);
-    if (s.aux) (void)hipEventRecord(s.join, s.aux);
-    if (ev0) (void)hipEventRecord(ev0, st);
     // One merged rev residue per k needs every j digit group's exponent
     // a-c-lj >= 0; otherwise track the groups separately.
     // primes >= TD_LAT_MIN enumerate their hits directly when the j digit
-    // groups merge; below, and for separate groups, the K-step lane kernel.
+    // groups merge, those from 2^16 below take the split k loop; for
+    // separate groups, every prime from 2^16 takes the K-step lane kernel.
     if (n && a - c >= DJ) {
         const uint32_t mid_blocks = std::min<uint64_t>(
             2048u, ((uint64_t)n * TD_MID_SPLIT + TD_BLOCK - 1u) / TD_BLOCK);
-        td_mid_kernel<DJ><<<mid_blocks, TD_BLOCK, 0, st>>>(
+        td_mid_kernel<DJ><<<mid_blocks, TD_BLOCK, 0, rs>>>(
             s.mid, s.ctrl, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits);
+    }
+    if (s.aux) (void)hipEventRecord(s.join, s.aux);
+    if (ev0) (void)hipEventRecord(ev0, st);
+    if (n && a - c >= DJ) {
         td_lattice_kernel<DJ><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
-            d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits, TD_LAT_MIN);
+            d_primes, n, a, b, c, K, J, d_bits, TD_LAT_MIN);
     } else if (n) {
         td_large_kernel<DJ, false><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
             d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits, TD_PMAX);
@@ -1398,6 +1289,11 @@ extern "C" int trial_div_check_error(void)
     const int e0 = td_check_error_code();
     const int e1 = td_check_error_noalloc();
     return e0 != 0 ? e0 : e1;
+}
+
+extern "C" uint32_t trial_div_lattice_min(void)
+{
+    return TD_LAT_MIN;
 }
 
 extern "C" int trial_div_launch_kj_noalloc(const uint32_t *d_primes, uint32_t n_primes,

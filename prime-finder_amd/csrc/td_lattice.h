@@ -188,8 +188,8 @@ PF_HD uint32_t td_first_hit(uint32_t c, uint32_t sig, uint32_t T, uint32_t sigin
 // l_k = 2..4 (the class l_k = 1, 9 values of r, is stepped in the setup).
 constexpr int TD_LAT_CONDS = 4;
 
-// Per-prime constants of the lattice path (the record the kernel keeps in
-// LDS while its lanes work through the conditions of 64 primes).
+// Per-prime constants of the lattice path: the start value, step and step
+// inverse of every condition to be searched.
 struct TdLatRec {
     uint32_t p, nprime;
     uint32_t un1, sigN, sinvN;       // N: start value, step, step^-1 (Montgomery)
@@ -341,8 +341,10 @@ PF_HD void td_lattice_run(const TdLatRec &r, int cond, const TdLatItem &it, uint
     }
 }
 
-// Every hit of one prime, condition by condition (the order of the
-// reference model; the kernel interleaves the same searches across lanes).
+// Every hit of one prime, condition by condition.  (Chaining the four
+// searches into one loop, so that a lane whose search is exhausted starts
+// its next condition without waiting for the rest of its wave, issued 5 %
+// fewer VALU instructions but ran 3 % longer on the 4096 x 4096 grid.)
 template <int DJ, class Mark>
 PF_HD void td_lattice_prime(uint32_t p, long long a, long long b, long long c, uint32_t K,
                             uint32_t J, Mark &mark)

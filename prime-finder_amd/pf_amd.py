@@ -51,6 +51,9 @@ def lib() -> C.CDLL:
         "trial_div_launch_kj_noalloc": (C.c_int, [vp, u32, ll, ll, ll, u32, u32, vp, vp]),
         "trial_div_check_error": (C.c_int, []),
         "trial_div_survivors": (C.c_int, [vp, u32, u32, vp, u32, C.POINTER(C.c_uint32), vp]),
+        "trial_div_lattice_min": (u32, []),
+        "trial_div_work": (C.c_int, [u32p, u32, ll, ll, ll, u32, u32, u32,
+                                     C.POINTER(C.c_uint64)]),
         "cuda_get_device_count": (C.c_int, []),
         "cuda_set_device": (C.c_int, [C.c_int]),
         "prime_list_load": (C.c_int, [C.POINTER(PrimeList), C.c_char_p]),
@@ -136,6 +139,22 @@ def trial_div_gpu_kj(primes: np.ndarray, a: int, b: int, c: int, K: int, J: int,
     if timing:
         return rc, out, (t.upload, t.kernel, t.download)
     return rc, out
+
+
+TD_WORK_FIELDS = ("prime", "modmul", "query", "level", "hit", "step", "kstep")
+
+
+def trial_div_work(primes: np.ndarray, a: int, b: int, c: int, K: int, J: int,
+                   threads: int = 1) -> dict:
+    """Algorithmic work of one launch on (a, b, c) over `primes`, counted on
+    the host (trial_div_work): lattice primes, Montgomery multiplications,
+    first-hit queries, continued-fraction levels, hits, one-digit-class
+    steps and k-loop prime x k steps."""
+    p = _u32(primes)
+    out = (C.c_uint64 * len(TD_WORK_FIELDS))()
+    if lib().trial_div_work(p.ctypes.data_as(u32p), len(p), a, b, c, K, J, threads, out) != 0:
+        raise ValueError("trial_div_work: invalid arguments")
+    return dict(zip(TD_WORK_FIELDS, (int(x) for x in out)))
 
 
 def survivors_gpu(d_bits, K: int, J: int, cap: int | None = None) -> np.ndarray:

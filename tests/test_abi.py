@@ -110,3 +110,30 @@ def test_cpp_runner_header_compiles():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", f"-I{INCLUDE}",
                         "-x", "c++", "-"], input=src, text=True, capture_output=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_trial_div_work_counts(pf):
+    """Host-side work counts used to rate the kernels: the lattice takes the
+    primes from trial_div_lattice_min() up, the k loop those from 2^16 below
+    it (or all from 2^16 when a - c < D_j), the counts do not depend on the
+    thread count, and invalid arguments are rejected."""
+    primes = numpy_primes(3_000_000)
+    lmin = int(pf.lib().trial_div_lattice_min())
+    assert 1 << 16 <= lmin <= 1 << 24
+    a, b, c, K, J = 60000, 21000, 100, 512, 700
+    w1 = pf.trial_div_work(primes, a, b, c, K, J, 1)
+    w4 = pf.trial_div_work(primes, a, b, c, K, J, 4)
+    assert w1 == w4
+    assert w1["prime"] == int((primes >= lmin).sum())
+    assert w1["kstep"] == K * int(((primes >= 1 << 16) & (primes < lmin)).sum())
+    # every lattice prime searches the N condition at least once, and each
+    # query that is not decided at once takes at least one level
+    assert w1["query"] >= w1["prime"] and w1["level"] >= w1["query"] - w1["hit"]
+    assert w1["modmul"] > 50 * w1["prime"]
+    # j digit groups that do not merge: every prime from 2^16 on the k loop
+    w = pf.trial_div_work(primes, 60000, 21000, 59998, K, J, 2)
+    assert w["prime"] == 0 and w["kstep"] == K * int((primes >= 1 << 16).sum())
+    with pytest.raises(ValueError):
+        pf.trial_div_work(primes, 100, 200, 10, K, J)
+    with pytest.raises(ValueError):
+        pf.trial_div_work(primes, 60000, 21000, 100, 0, J)
