@@ -45,16 +45,20 @@ DJ = 4
 #   hit     k and j decoded from a hit and the bit marked: 12;
 #   step    one direct step of the one-digit rev class: modular subtraction
 #           and compare, 4;
+#   gapstep one return-time step from a hit to the next (two compares, the
+#           selects of the step and of the landing residue, two adds, the
+#           end test): 8;
 #   kstep   one k of the split k loop (primes from 2^16 below the lattice
 #           range, beside the lattice kernel on the second stream): two
 #           residues advanced and compared, 8.
-TD_OPS_PER_UNIT = {"modmul": 8, "level": 45, "query": 10, "hit": 12, "step": 4, "kstep": 8}
+TD_OPS_PER_UNIT = {"modmul": 8, "level": 45, "query": 10, "hit": 12, "step": 4, "gapstep": 8,
+                   "kstep": 8}
 INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12      # 39.32 Tops/s (256 CUs, wave64, 2.4 GHz)
-# HBM bytes per launch of the lane kernel on this workload, measured with
+# HBM bytes per launch of the lattice kernel on this workload, measured with
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 256 MiB copy kernel's
-# readings: x2.0 and x1.0): 23.3 MB read (the prime list) + 535.5 MB written
-# (memory-side atomic ORs of the ~1.7e7 hits).
-TD_TRAFFIC_BYTES = 23.3e6 + 535.5e6
+# readings: x2.0 and x1.0): 23.21 MB read (the prime list) + 417.82 MB
+# written (memory-side atomic ORs of the hits).
+TD_TRAFFIC_BYTES = 23.21e6 + 417.82e6
 # Same for one batched squaring step of the 50 000-digit Miller-Rabin batch
 # (256 bigints, 2^14 transforms): 91.7 MB read + 14.4 MB written.
 MR_TRAFFIC_BYTES_PER_STEP = 91.7e6 + 14.4e6
@@ -685,7 +689,7 @@ def main() -> int:
                      "traffic": TD_TRAFFIC_BYTES,
                      "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE); "
                                      "algorithmic 25.1e6 (prime list + bitmap), the rest are "
-                                     "hit atomics; 0.11 TB/s, far below the 8 TB/s roof",
+                                     "hit atomics; 0.25 TB/s, far below the 8 TB/s roof",
                      "kernel": "td_lattice_kernel<4>",
                      "kernel_avg_ms": avg_lane_s * 1e3,
                      "ops_per_launch": alg_ops,

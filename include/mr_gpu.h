@@ -195,6 +195,29 @@ int mr_sched_results(mr_sched_t *s, mr_result_t *out, int cap);
 int mr_sched_stats(const mr_sched_t *s, mr_sched_stats_t *out);
 void mr_sched_destroy(mr_sched_t *s);
 
+/* Per-bucket choices, replacing the reference's configure-time parameters
+ * (params.cmake:6-64, miller_rabin_gpu/config.h.in:11-95) with values a
+ * tuner measured on the device.  mr_sched_set_bucket: candidates of
+ * transform length 2^log_len (11..15) run in chunks of `chunk` squaring
+ * steps in a pool of `slots` slots (0: the scheduler-wide chunk and
+ * max_active); when max_active was 0 at creation, the active cap rises to
+ * the largest bucket's slots.  Must precede the bucket's first admission
+ * (its pool's graph holds the chunk): returns 0, or -1 (invalid values, or
+ * the pool exists already).
+ *
+ * A tuning table is a JSON object whose "buckets" array holds objects of
+ * integer fields "log_len", "chunk" and "slots_per_cu" (slots per compute
+ * unit; other fields are ignored, nested values are not allowed inside a
+ * bucket), as prime-finder_amd/tune_mr.py writes it.  mr_tuning_read fills
+ * up to cap entries and returns the number of buckets in the file, or -1
+ * (unreadable or malformed); no GPU call.  mr_sched_load_tuning applies a
+ * table to s (slots = slots_per_cu x the device's compute units) and returns
+ * the number of buckets set, or -1 (nothing set).
+ * No counterpart in the reference. */
+int mr_sched_set_bucket(mr_sched_t *s, int log_len, int chunk, int slots);
+int mr_tuning_read(const char *path, int *log_len, int *chunk, int *slots_per_cu, int cap);
+int mr_sched_load_tuning(mr_sched_t *s, const char *path);
+
 /* Checkpoint and resume of a serving scheduler.  The reference's master
  * dumps its pending Phase-2 queue as "a b c k j" lines (master.c:279-369,
  * checkpoint_save_phase2 / checkpoint_load_phase2) and restarts every test

@@ -2011,6 +2011,7 @@ int64_t slot_bytes(int logm)
 
 struct Pool {
     int logm = 0, n = 0;
+    int chunk = 0;                  // squaring steps per chunk (graph launch)
     mr_ctx *x = nullptr;            // x->st is the admin stream
     hipStream_t st_run = nullptr;
     hipEvent_t ev_run = nullptr, ev_adm = nullptr;
@@ -2042,6 +2043,13 @@ struct Pool {
 
 struct mr_sched {
     int device = 0, n_bases = 0, max_active = 0, chunk = 256;
+    int cus = 0;                   // compute units of the device
+    int pool_slots = 0;            // slots of a pool without a bucket choice
+    bool auto_active = false;      // max_active defaulted: follows the buckets' slots
+    // per-bucket choices (0: the scheduler-wide chunk / max_active), set
+    // before the bucket's pool exists (mr_sched_set_bucket, a tuning table)
+    int bucket_chunk[MAX_LOGM + 1] = {};
+    int bucket_slots[MAX_LOGM + 1] = {};
     int64_t budget = 0;
     int64_t hbm = 0;               // device memory held by the pools (all slots)
     int64_t job_bytes = 0;         // footprint of the admitted jobs (their slots)
@@ -2083,13 +2091,14 @@ int pool_new(mr_sched *S, int logm, Pool **out)
 {
     const int64_t per = slot_bytes(logm);
     int64_t room = (S->budget - S->hbm) / per;
-    int n = S->max_active;
+    int n = std::min(S->bucket_slots[logm] ? S->bucket_slots[logm] : S->pool_slots, S->max_active);
     if (room < n) n = (int)room;
     if (n < 1) return 1;                 // no room under the budget now
     Pool *p = new (std::nothrow) Pool;
     if (!p) return -2;
     p->logm = logm;
     p->n = n;
+    p->chunk = S->bucket_chunk[logm] ? S->bucket_chunk[logm] : S->chunk;
     p->x = ctx_new(1, n, S->device);
     int rc = p->x ? 0 : -2;
     // numbers of up to (2^logm - 2) / 2 limbs (24 lmax - 1 bits): the pool's
@@ -2194,7 +2203,7 @@ int pool_graph(mr_sched *S, Pool *p)
     Dev Dg = p->x->d;
     Dg.list = nullptr;
     Dg.run = p->d_run;
-    const int chunk = S->chunk, n = p->n;
+    const int chunk = p->chunk, n = p->n;
     MR_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     hipError_t e = hipMemcpyAsync(p->d_run, p->h_run, (size_t)n, hipMemcpyHostToDevice, st);
     if (e == hipSuccess) {
@@ -2230,7 +2239,7 @@ int launch_chunk(mr_sched *S, Pool *p)
     if (!p->gexec) { int rc = pool_graph(S, p); if (rc) return rc; }
     MR_CHECK(hipGraphLaunch(p->gexec, p->st_run));
     MR_CHECK(hipEventRecord(p->ev_run, p->st_run));
-    const int chunk = S->chunk;
+    const int chunk = p->chunk;
     p->run_busy = true;
     S->chunks++;
     S->squarings += (int64_t)nr * chunk;
@@ -2383,14 +2392,15 @@ extern "C" mr_sched_t *mr_sched_create(const uint32_t *bases, int n_bases, int d
         if (bases[i] < 2u || bases[i] >= (1u << 24)) return nullptr;
     if (device < 0 && hipGetDevice(&device) != hipSuccess) return nullptr;
     if (hipSetDevice(device) != hipSuccess) return nullptr;
-    if (max_active == 0) {
-        int cus = 0;
-        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
-            return nullptr;
-        max_active = cus;
-    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return nullptr;
     mr_sched *S = new (std::nothrow) mr_sched;
     if (!S) return nullptr;
+    S->auto_active = max_active == 0;
+    if (max_active == 0) max_active = cus;
+    S->cus = cus;
+    S->pool_slots = max_active;
     S->device = device;
     S->n_bases = n_bases;
     S->bases.assign(bases, bases + n_bases);
@@ -2681,6 +2691,94 @@ extern "C" int mr_sched_results(mr_sched_t *S, mr_result_t *out, int cap)
     while (k < cap && S->done_head < S->done.size()) out[k++] = S->done[S->done_head++];
     if (S->done_head == S->done.size()) { S->done.clear(); S->done_head = 0; }
     return k;
+}
+
+extern "C" int mr_sched_set_bucket(mr_sched_t *S, int log_len, int chunk, int slots)
+{
+    if (!S || log_len < MIN_LOGM || log_len > MAX_LOGM || chunk < 0 || chunk > (1 << 20) ||
+        slots < 0 || slots > (1 << 20))
+        return -1;
+    if (S->pools[log_len]) return -1;   // the pool is built: its graph holds the chunk
+    S->bucket_chunk[log_len] = chunk;
+    S->bucket_slots[log_len] = slots;
+    if (S->auto_active && slots > S->max_active) S->max_active = slots;
+    return 0;
+}
+
+namespace {
+
+// The "buckets" array of a tuning table: objects of integer fields only,
+// e.g. {"log_len": 14, "chunk": 256, "slots_per_cu": 1, ...}.  Other
+// top-level keys (measurements, device) are skipped.  Returns the number of
+// buckets, or -1 when the array is missing or malformed.
+int parse_tuning(const std::string &t, int *log_len, int *chunk, int *spc, int cap)
+{
+    const size_t key = t.find("\"buckets\"");
+    if (key == std::string::npos) return -1;
+    size_t i = t.find_first_not_of(" \t\r\n", key + 9);
+    if (i == std::string::npos || t[i] != ':') return -1;
+    i = t.find_first_not_of(" \t\r\n", i + 1);
+    if (i == std::string::npos || t[i] != '[') return -1;
+    int n = 0;
+    for (i++;;) {
+        i = t.find_first_not_of(" \t\r\n,", i);
+        if (i == std::string::npos) return -1;
+        if (t[i] == ']') return n;
+        if (t[i] != '{') return -1;
+        const size_t end = t.find('}', i);
+        if (end == std::string::npos) return -1;
+        const std::string obj = t.substr(i + 1, end - i - 1);
+        if (obj.find_first_of("{[") != std::string::npos) return -1;
+        long v[3] = {-1, -1, -1};
+        const char *names[3] = {"\"log_len\"", "\"chunk\"", "\"slots_per_cu\""};
+        for (int f = 0; f < 3; f++) {
+            const size_t k = obj.find(names[f]);
+            if (k == std::string::npos) return -1;
+            const size_t c = obj.find(':', k);
+            if (c == std::string::npos) return -1;
+            char *stop = nullptr;
+            v[f] = strtol(obj.c_str() + c + 1, &stop, 10);
+            if (stop == obj.c_str() + c + 1) return -1;
+        }
+        if (v[0] < MIN_LOGM || v[0] > MAX_LOGM || v[1] < 0 || v[1] > (1 << 20) || v[2] < 0 || v[2] > 64)
+            return -1;
+        if (n < cap) {
+            log_len[n] = (int)v[0];
+            chunk[n] = (int)v[1];
+            spc[n] = (int)v[2];
+        }
+        n++;
+        i = end + 1;
+    }
+}
+
+} // namespace
+
+extern "C" int mr_tuning_read(const char *path, int *log_len, int *chunk, int *slots_per_cu,
+                              int cap)
+{
+    if (!path || cap < 0 || (cap > 0 && (!log_len || !chunk || !slots_per_cu))) return -1;
+    FILE *f = fopen(path, "rb");
+    if (!f) return -1;
+    std::string t;
+    char buf[4096];
+    size_t got;
+    while ((got = fread(buf, 1, sizeof buf, f)) > 0 && t.size() < (1u << 22)) t.append(buf, got);
+    fclose(f);
+    return parse_tuning(t, log_len, chunk, slots_per_cu, cap);
+}
+
+extern "C" int mr_sched_load_tuning(mr_sched_t *S, const char *path)
+{
+    if (!S) return -1;
+    int lg[MAX_LOGM + 1], ch[MAX_LOGM + 1], spc[MAX_LOGM + 1];
+    const int n = mr_tuning_read(path, lg, ch, spc, MAX_LOGM + 1);
+    if (n < 0 || n > MAX_LOGM + 1) return -1;
+    for (int i = 0; i < n; i++)
+        if (S->pools[lg[i]]) return -1;          // nothing set: a pool is built already
+    for (int i = 0; i < n; i++)
+        if (mr_sched_set_bucket(S, lg[i], ch[i], spc[i] * S->cus) != 0) return -1;
+    return n;
 }
 
 extern "C" int mr_sched_stats(const mr_sched_t *S, mr_sched_stats_t *o)

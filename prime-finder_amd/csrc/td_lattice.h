@@ -182,6 +182,41 @@ PF_HD uint32_t td_first_hit(uint32_t c, uint32_t sig, uint32_t T, uint32_t sigin
     return td_fh_result(q, siginvM, m, y);
 }
 
+// ---- return times of the window ------------------------------------------
+//
+// Three-gap structure of the returns of a rotation to an interval: with
+// e(n) = n sig mod p, let a = min{n >= 1 : e(n) < T} with alpha = e(a), and
+// b = min{n >= 1 : e(n) > p - T} with beta = p - e(b).  Then alpha + beta >= T,
+// and from any v in [0, T) the first n >= 1 with (v + e(n)) mod p < T is
+//   a      (landing at v + alpha)          if v < T - alpha,
+//   b      (landing at v - beta)           if v >= beta,
+//   a + b  (landing at v + alpha - beta)   otherwise.
+// So after a hit of a search every further hit is one compare-and-add step
+// instead of a first-hit query.  a and alpha are the first hit of the
+// sequence sig + t sig below T; b and beta that of (sig + T - 1) + t sig
+// below T - 1 (e(n) > p - T exactly when e(n) + T - 1 wraps below T - 1).
+struct TdGaps {
+    uint32_t a, alpha, b, beta;
+};
+
+// The return times of [0, T) under sig (T >= 2); false when a search fails
+// (a composite list entry).
+PF_HD bool td_gaps(uint32_t sig, uint32_t T, uint32_t siginvM, const PfMont &m, TdGaps &g)
+{
+    uint32_t y;
+    const uint32_t xa = td_first_hit(sig, sig, T, siginvM, m, y);
+    if (xa == 0xFFFFFFFFu) return false;
+    g.a = xa + 1u;
+    g.alpha = y;
+    uint32_t cb = sig + (T - 1u);
+    cb = cb >= m.p ? cb - m.p : cb;
+    const uint32_t xb = td_first_hit(cb, sig, T - 1u, siginvM, m, y);
+    if (xb == 0xFFFFFFFFu) return false;
+    g.b = xb + 1u;
+    g.beta = T - 1u - y;
+    return true;
+}
+
 // ---- per-prime setup and conditions -----------------------------------
 //
 // Conditions of one prime: cond 0 is N, cond 1..3 the rev classes
@@ -317,8 +352,10 @@ PF_HD void td_lattice_setup(uint32_t p, long long a, long long b, long long c, u
     }
 }
 
-// Every hit of one condition, in order of t: a first-hit query from t0,
-// the hit marked, the next query from t + 1.
+// Every hit of one condition, in order of t: the first hit by a first-hit
+// query, then the window's return times (td_gaps, two queries) step from hit
+// to hit.  A search expecting fewer than two hits (W T < 2 p) looks for its
+// second hit by a query too, and takes the return times only from a third.
 template <int DJ, class Mark>
 PF_HD void td_lattice_run(const TdLatRec &r, int cond, const TdLatItem &it, uint32_t K, uint32_t J,
                           Mark &mark)
@@ -327,17 +364,39 @@ PF_HD void td_lattice_run(const TdLatRec &r, int cond, const TdLatItem &it, uint
     m.p = r.p;
     m.nprime = r.nprime;
     m.one = m.r2 = 0u;                  // (not used by the queries)
-    uint32_t cc = it.c0, t0 = 0u;
-    for (;;) {
-        uint32_t y;
+    const bool many = (uint64_t)it.W * it.T >= 2ull * r.p;
+    uint32_t cc = it.c0, t0 = 0u, t = 0u, y = 0u;
+    for (int n = 0;; n++) {
         const uint32_t x = td_first_hit(cc, it.sig, it.T, it.sinv, m, y);
-        if (x >= it.W - t0) break;                          // (no overflow: t0 < W)
-        const uint32_t t = t0 + x;
+        if (x >= it.W - t0) return;                         // (no overflow: t0 < W)
+        t = t0 + x;
         td_lattice_mark<DJ>(cond, t, y, K, J, mark);
         t0 = t + 1u;
-        if (t0 >= it.W) break;
+        if (t0 >= it.W) return;
+        if (it.T >= 2u && (many || n == 1)) break;
         cc = y + it.sig;
         cc = cc >= r.p ? cc - r.p : cc;
+    }
+    TdGaps g;
+    if (!td_gaps(it.sig, it.T, it.sinv, m, g)) return;     // (composite entry: stop)
+    // steps capped at W (any step reaching W ends the search): t stays below 2 W
+    const uint32_t da = g.a < it.W ? g.a : it.W;
+    const uint32_t db = g.b < it.W ? g.b : it.W;
+    const uint32_t dab = (uint64_t)g.a + g.b < it.W ? g.a + g.b : it.W;
+    for (;;) {
+        TD_LAT_COUNT(gapstep);
+        if (y < it.T - g.alpha) {
+            t += da;
+            y += g.alpha;
+        } else if (y >= g.beta) {
+            t += db;
+            y -= g.beta;
+        } else {
+            t += dab;
+            y = y + g.alpha - g.beta;
+        }
+        if (t >= it.W) return;
+        td_lattice_mark<DJ>(cond, t, y, K, J, mark);
     }
 }
 

@@ -141,7 +141,7 @@ def trial_div_gpu_kj(primes: np.ndarray, a: int, b: int, c: int, K: int, J: int,
     return rc, out
 
 
-TD_WORK_FIELDS = ("prime", "modmul", "query", "level", "hit", "step", "kstep")
+TD_WORK_FIELDS = ("prime", "modmul", "query", "level", "hit", "step", "kstep", "gapstep")
 
 
 def trial_div_work(primes: np.ndarray, a: int, b: int, c: int, K: int, J: int,
@@ -149,7 +149,7 @@ def trial_div_work(primes: np.ndarray, a: int, b: int, c: int, K: int, J: int,
     """Algorithmic work of one launch on (a, b, c) over `primes`, counted on
     the host (trial_div_work): lattice primes, Montgomery multiplications,
     first-hit queries, continued-fraction levels, hits, one-digit-class
-    steps and k-loop prime x k steps."""
+    steps, k-loop prime x k steps and return-time steps between hits."""
     p = _u32(primes)
     out = (C.c_uint64 * len(TD_WORK_FIELDS))()
     if lib().trial_div_work(p.ctypes.data_as(u32p), len(p), a, b, c, K, J, threads, out) != 0:
@@ -267,6 +267,9 @@ def _mr_sigs(L):
         "mr_sched_destroy": (None, [vp]),
         "mr_sched_checkpoint": (i32, [vp, C.c_char_p]),
         "mr_sched_restore": (i32, [vp, C.c_char_p]),
+        "mr_sched_set_bucket": (i32, [vp, i32, i32, i32]),
+        "mr_tuning_read": (i32, [C.c_char_p, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), i32]),
+        "mr_sched_load_tuning": (i32, [vp, C.c_char_p]),
         "mr_test_batch": (i32, [C.POINTER(MrCandidate), i32, C.POINTER(u32), i32, i32,
                                 C.POINTER(C.c_uint8), C.POINTER(C.c_int32)]),
     }
@@ -310,17 +313,53 @@ def _cands(cands):
     return arr
 
 
+# The tuning table tune_mr.py writes for this package (per-bucket chunk and
+# slots measured on the device).
+TUNING_PATH = Path(__file__).resolve().parent / "mr_tuning.json"
+
+
+def tuning_read(path) -> list[dict]:
+    """The buckets of a tuning table as the library parses it
+    (mr_tuning_read; no GPU call)."""
+    L = mr_lib()
+    cap = 16
+    lg, ch, spc = (C.c_int32 * cap)(), (C.c_int32 * cap)(), (C.c_int32 * cap)()
+    n = L.mr_tuning_read(os.fsencode(str(path)), lg, ch, spc, cap)
+    if n < 0:
+        raise ValueError(f"mr_tuning_read rejected {path}")
+    return [{"log_len": lg[i], "chunk": ch[i], "slots_per_cu": spc[i]} for i in range(min(n, cap))]
+
+
 class MrScheduler:
     """Continuous batching over mixed-size candidates (mr_sched_* C ABI)."""
 
     def __init__(self, bases, device: int = 0, max_active: int = 0, chunk: int = 0,
-                 hbm_budget: int = 0):
+                 hbm_budget: int = 0, tuning=None):
+        """tuning: a tuning table (path) whose per-bucket chunk and slots
+        apply (mr_sched_load_tuning); "default" = TUNING_PATH when it exists."""
         L = mr_lib()
         self.L = L
         b = (C.c_uint32 * len(bases))(*bases)
         self.ptr = L.mr_sched_create(b, len(bases), device, max_active, chunk, hbm_budget)
         if not self.ptr:
             raise RuntimeError("mr_sched_create failed")
+        self.tuned = 0
+        if tuning == "default":
+            tuning = TUNING_PATH if TUNING_PATH.exists() else None
+        if tuning is not None:
+            self.tuned = self.load_tuning(tuning)
+
+    def set_bucket(self, log_len: int, chunk: int = 0, slots: int = 0):
+        """Chunk and pool slots for the bucket of transform length 2^log_len
+        (before its first admission)."""
+        if self.L.mr_sched_set_bucket(self.ptr, log_len, chunk, slots) != 0:
+            raise ValueError(f"mr_sched_set_bucket({log_len}, {chunk}, {slots}) rejected")
+
+    def load_tuning(self, path) -> int:
+        n = self.L.mr_sched_load_tuning(self.ptr, os.fsencode(str(path)))
+        if n < 0:
+            raise ValueError(f"mr_sched_load_tuning rejected {path}")
+        return n
 
     def submit(self, cands, tags=None):
         arr = _cands(cands)

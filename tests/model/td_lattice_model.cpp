@@ -31,3 +31,38 @@ extern "C" int lattice_sieve(const uint32_t *primes, uint32_t n, long long a, lo
     }
     return 0;
 }
+
+// The return-time rule of td_gaps against brute force: for every v in
+// [0, T), the first n >= 1 with (v + n sig) mod p < T and its landing
+// residue.  p an odd prime, 1 <= sig < p, 2 <= T < p.  Returns the number of
+// v where the rule and the brute force differ (-1: td_gaps failed).
+extern "C" int gaps_rule_mismatches(uint32_t p, uint32_t sig, uint32_t T)
+{
+    const PfMont m = pf_mont_init(p);
+    const uint32_t siginvM = pf_mont_pow(pf_mont_in(sig, m), (uint64_t)p - 2u, m);
+    TdGaps g;
+    if (!td_gaps(sig, T, siginvM, m, g)) return -1;
+    if ((uint64_t)g.alpha + g.beta < T) return -2;
+    int bad = 0;
+    for (uint32_t v = 0; v < T; v++) {
+        uint64_t n = 1;
+        uint64_t w = ((uint64_t)v + sig) % p;
+        while (w >= T) {
+            n++;
+            w = (w + sig) % p;
+        }
+        uint64_t rn, rw;
+        if (v < T - g.alpha) {
+            rn = g.a;
+            rw = v + g.alpha;
+        } else if (v >= g.beta) {
+            rn = g.b;
+            rw = v - g.beta;
+        } else {
+            rn = (uint64_t)g.a + g.b;
+            rw = (uint64_t)v + g.alpha - g.beta;
+        }
+        if (rn != n || rw != w) bad++;
+    }
+    return bad;
+}

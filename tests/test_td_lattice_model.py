@@ -31,6 +31,8 @@ def model(tmp_path_factory):
     L.lattice_sieve.argtypes = [u32p, C.c_uint32, C.c_longlong, C.c_longlong, C.c_longlong,
                                 C.c_uint32, C.c_uint32, u32p]
     L.lattice_sieve.restype = C.c_int
+    L.gaps_rule_mismatches.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
+    L.gaps_rule_mismatches.restype = C.c_int
     return L
 
 
@@ -105,3 +107,20 @@ def test_composite_entries_terminate(model):
     comps = np.array([3 * 5 * 7 * 11 * 13 * 17, 1000001, 2147483647 * 0 + 2147483645,
                       10 ** 8 + 1, 65535 * 3], np.uint32)
     run_model(model, comps, 60000, 30000, 100, 4096, 4096)
+
+
+def test_return_time_rule_brute_force(model):
+    """The three-gap rule the kernel steps by after a hit (td_gaps): from
+    every v of the window [0, T), the first return of v + n sig (mod p) and
+    where it lands, against brute force, on random (p, sig, T) with small p,
+    windows from 2 to p - 1, steps next to 1 and p - 1."""
+    rng = random.Random(3)
+    primes = [p for p in range(11, 4000) if _isprime(p)]
+    cases = 0
+    for _ in range(300):
+        p = rng.choice(primes)
+        sig = rng.choice([1, 2, p - 1, p - 2, rng.randrange(1, p)])
+        T = rng.choice([2, 3, p - 1, rng.randrange(2, p)])
+        assert model.gaps_rule_mismatches(p, sig, T) == 0, (p, sig, T)
+        cases += 1
+    assert cases == 300
