@@ -106,7 +106,8 @@ int trial_div_check_error(void);
  * reference; it is for callers that cannot allocate (code under HIP graph
  * capture, tight serving loops, harnesses that forbid allocation).  Its
  * scratch (small-prime list and setup table, control words, rev(j) decode
- * table, row-kernel work items, mid-range prime list: 11.37 MB) is one
+ * table, row-kernel work items, mid-range prime list, class masks of the
+ * small primes' word fills: 11.75 MB) is one
  * static device array that
  * each GPU the library runs on holds until the process ends; every kernel
  * runs in order on `stream`.  Limits: K, J in 1..9999 (-1 and a message

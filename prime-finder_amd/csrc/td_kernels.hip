@@ -69,9 +69,38 @@ enum {
     TT_E = 9,    // 16 words: E[(lk-1)*4 + (lj-1)]
 };
 
-// ctrl words
+// Word-pattern fills (the reference's small-prime Phase 0, sieve_table.c:93-243
+// and sieve.c:4-31, done per row on the device): for the odd primes below 32
+// other than 5 a 32-bit word of a row holds several hits of each condition,
+// so the merged-rev row kernel ORs whole words from per-class masks instead
+// of marking hits one by one.  For p >= 32 a period spans more than a word
+// and the row kernel's strided marks are already one store per period.
+constexpr int      TD_NFILL        = 9;
+constexpr uint32_t TD_FILL_CLASSES = 153;   // 3 + 7 + 11 + ... + 31: classes of all fill primes
+__constant__ uint32_t c_fill_p[TD_NFILL]   = {3u, 7u, 11u, 13u, 17u, 19u, 23u, 29u, 31u};
+__constant__ uint32_t c_fill_off[TD_NFILL] = {0u, 3u, 10u, 21u, 34u, 51u, 70u, 93u, 122u};
+
+// Index of p among the fill primes, or -1.
+__host__ __device__ __forceinline__ int td_fill_index(uint32_t p)
+{
+    switch (p) {
+    case 3u: return 0;  case 7u: return 1;  case 11u: return 2; case 13u: return 3;
+    case 17u: return 4; case 19u: return 5; case 23u: return 6; case 29u: return 7;
+    case 31u: return 8;
+    default: return -1;
+    }
+}
+
+// ctrl words: CT_FILL has bit f set when fill prime f is in the list, and
+// CT_FILL_IDX + f is the setup-table entry of (one copy of) that prime.
 enum { CT_SMALL_COUNT = 0, CT_ERROR = 1, CT_ITEMS = 2, CT_ITEM_OVF = 3, CT_MID_COUNT = 4,
-       CT_WORDS = 5 };
+       CT_FILL = 5, CT_FILL_IDX = 6, CT_WORDS = CT_FILL_IDX + TD_NFILL };
+
+// Class-mask table of the fill primes for a row of J bits (qw = ceil(J/32)
+// words per mask): mask (kind, f, r) at word (kind * TD_FILL_CLASSES +
+// c_fill_off[f] + r) * qw; kind 0 holds the j with j mod p == r, kind 1 the
+// j with R(j) mod p == r (R(j) = rev(j) padded with zeros to D_j digits).
+constexpr uint32_t TD_FILL_WORDS = 2u * TD_FILL_CLASSES * ((9999u + 31u) / 32u);
 
 // Primes in [TD_SMALL_LIMIT, mid limit) are compacted into a list of at most
 // TD_MID_CAP entries (every prime of [2^16, 2^24) fits) for the split k-loop
@@ -115,6 +144,7 @@ __global__ void td_reset_kernel(uint32_t *ctrl)
     if (threadIdx.x == 0) {
         ctrl[CT_SMALL_COUNT] = 0u;
         ctrl[CT_MID_COUNT] = 0u;
+        ctrl[CT_FILL] = 0u;
     }
 }
 
@@ -159,7 +189,7 @@ void td_split_kernel(const uint32_t *__restrict__ primes, uint32_t n,
 
 __global__ __launch_bounds__(TD_BLOCK)
 void td_small_setup_kernel(const uint32_t *__restrict__ small,
-                           const uint32_t *__restrict__ ctrl,
+                           uint32_t *__restrict__ ctrl,
                            long long a, long long b, long long c,
                            uint32_t *__restrict__ tbl)
 {
@@ -199,6 +229,12 @@ void td_small_setup_kernel(const uint32_t *__restrict__ small,
                 if (kres[lk] != 0u && jinv[lj] != 0u) flags |= 1u << (1 + lk * 4 + lj);
             }
         e[TT_FLAGS * TD_SMALL_CAP] = flags;
+        const int f = td_fill_index(p);
+        if (f >= 0) {
+            // copies of one prime have equal entries, so any writer's index serves
+            ctrl[CT_FILL_IDX + f] = i;
+            atomicOr(&ctrl[CT_FILL], 1u << f);
+        }
     }
 }
 
@@ -273,7 +309,8 @@ void td_rows_kernel(const uint32_t *__restrict__ tbl, const uint32_t *__restrict
 // ceil(ceil(J/p) / TD_CH) normal-solve parts when the solve is active and
 // ceil(ceil((10^DJ - 1)/p) / TD_CH) merged-rev parts; p = 2 and 5 (whose
 // powers of ten vanish, so digit groups are gated one by one) keep one
-// whole-prime item.  Item = index | kind << 16 | part << 18.
+// whole-prime item, and the fill primes none (the row kernel ORs their
+// class masks).  Item = index | kind << 16 | part << 18.
 template <int DJ>
 __global__ __launch_bounds__(1024)
 void td_items_kernel(const uint32_t *__restrict__ tbl, uint32_t *__restrict__ ctrl, uint32_t J,
@@ -287,6 +324,7 @@ void td_items_kernel(const uint32_t *__restrict__ tbl, uint32_t *__restrict__ ct
     auto parts = [&](uint32_t i, uint32_t &nn, uint32_t &nr) {
         const uint32_t p = tbl[TT_P * TD_SMALL_CAP + i];
         if (p == 2u || p == 5u) { nn = 0; nr = 0; return true; }
+        if (td_fill_index(p) >= 0) { nn = 0; nr = 0; return false; }   // word fills
         const uint32_t flags = tbl[TT_FLAGS * TD_SMALL_CAP + i];
         nn = (flags & 1u) ? ((J + p - 1u) / p + TD_CH - 1u) / TD_CH : 0u;
         nr = ((lim - 1u + p - 1u) / p + TD_CH - 1u) / TD_CH;
@@ -323,23 +361,113 @@ void td_items_kernel(const uint32_t *__restrict__ tbl, uint32_t *__restrict__ ct
     }
 }
 
+// Class masks of the fill primes for rows of J bits (layout at
+// TD_FILL_WORDS).  One lane per mask word.
+template <int DJ>
+__global__ __launch_bounds__(TD_BLOCK)
+void td_fill_table_kernel(uint32_t J, uint32_t *__restrict__ tab)
+{
+    const uint32_t qw = (J + 31u) >> 5;
+    const uint32_t total = 2u * TD_FILL_CLASSES * qw;
+    for (uint32_t idx = blockIdx.x * TD_BLOCK + threadIdx.x; idx < total;
+         idx += gridDim.x * TD_BLOCK) {
+        const uint32_t row = idx / qw, q = idx - row * qw;
+        const uint32_t kind = row >= TD_FILL_CLASSES ? 1u : 0u;
+        const uint32_t cls = row - kind * TD_FILL_CLASSES;
+        int f = 0;
+        while (f + 1 < TD_NFILL && c_fill_off[f + 1] <= cls) f++;
+        const uint32_t p = c_fill_p[f], r = cls - c_fill_off[f];
+        uint32_t w = 0;
+        for (uint32_t i = 0; i < 32u; i++) {
+            const uint32_t j = q * 32u + i + 1u;
+            if (j > J) break;
+            uint32_t x = j;
+            if (kind) {
+                const int lj = td_ndigits(j);
+                x = td_reverse(j, lj) * td_pow10(DJ - lj);
+            }
+            if (x % p == r) w |= 1u << i;
+        }
+        tab[idx] = w;
+    }
+}
+
 // Row kernel with merged rev residues (a - c >= DJ): lanes spread over
 // (row, work item) instead of (row, prime), so the many marks of the
 // smallest primes are shared by many lanes.  rdec is the DJ-digit decode
-// table.  Falls back to whole primes if the item list overflowed.
+// table.  Falls back to whole primes if the item list overflowed.  The
+// fill primes present in the list (ctrl[CT_FILL]) are marked first, a word
+// at a time: per row and fill prime the N condition's hits are the j of one
+// class mod p (j = u + 1 + t p) and the rev condition's the j whose R(j) is
+// of one class, so the row's words are ORs of the table's class masks,
+// shifted to the row's bit offset.
 template <int DJ>
 __global__ __launch_bounds__(TD_BLOCK)
 void td_rows_bal_kernel(const uint32_t *__restrict__ tbl, const uint32_t *__restrict__ ctrl,
                         const uint32_t *__restrict__ items, const uint16_t *__restrict__ rdec,
+                        const uint32_t *__restrict__ ftab,
                         uint32_t K, uint32_t J, uint32_t *__restrict__ bits)
 {
     extern __shared__ uint32_t lds[];
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    __shared__ uint32_t fmask_at[TD_ROWS][2][TD_NFILL];   // class-mask word offsets, or NONE
     const uint32_t k0 = blockIdx.x * TD_ROWS + 1u;
     const uint32_t k_last = min(k0 + TD_ROWS - 1u, K);
     const uint32_t w0 = ((k0 - 1u) * J) >> 5;
     const uint32_t nw = ((k_last * J - 1u) >> 5) - w0 + 1u;
     for (uint32_t w = threadIdx.x; w < nw; w += TD_BLOCK) lds[w] = 0u;
+    const uint32_t fill = ctrl[CT_FILL];
+    const uint32_t qw = (J + 31u) >> 5;
+    if (threadIdx.x < TD_ROWS * TD_NFILL) {
+        const uint32_t r = threadIdx.x / TD_NFILL, f = threadIdx.x - r * TD_NFILL;
+        const uint32_t kk = k0 + r;
+        uint32_t at_n = NONE, at_r = NONE;
+        if (kk <= K && ((fill >> f) & 1u)) {
+            const uint32_t *e = tbl + ctrl[CT_FILL_IDX + f];
+            const uint32_t p = e[TT_P * TD_SMALL_CAP], magic = e[TT_MAGIC * TD_SMALL_CAP];
+            const uint32_t flags = e[TT_FLAGS * TD_SMALL_CAP];
+            if (flags & 1u) {
+                const uint32_t u = pf_submod(e[TT_UN0 * TD_SMALL_CAP],
+                                             td_mod(kk * e[TT_B * TD_SMALL_CAP], p, magic), p);
+                const uint32_t cls = u + 1u == p ? 0u : u + 1u;
+                at_n = (c_fill_off[f] + cls) * qw;
+            }
+            const int lk = td_ndigits(kk);
+            if ((flags >> (1 + (lk - 1) * 4 + (DJ - 1))) & 1u) {
+                const uint32_t ev = e[(TT_E + (lk - 1) * 4 + (DJ - 1)) * TD_SMALL_CAP];
+                const uint32_t u = pf_submod(e[(TT_CM + DJ - 1) * TD_SMALL_CAP],
+                                             td_mod(td_reverse(kk, lk) * ev, p, magic), p);
+                const uint32_t cls = u + 1u == p ? 0u : u + 1u;
+                at_r = (TD_FILL_CLASSES + c_fill_off[f] + cls) * qw;
+            }
+        }
+        fmask_at[r][0][f] = at_n;
+        fmask_at[r][1][f] = at_r;
+    }
     __syncthreads();
+
+    if (fill) {
+        for (uint32_t t = threadIdx.x; t < TD_ROWS * qw; t += TD_BLOCK) {
+            const uint32_t r = t / qw, q = t - r * qw;
+            if (k0 + r > K) break;
+            uint32_t m = 0;
+#pragma unroll
+            for (int f = 0; f < TD_NFILL; f++) {
+                const uint32_t an = fmask_at[r][0][f], ar = fmask_at[r][1][f];
+                if (an != NONE) m |= ftab[an + q];
+                if (ar != NONE) m |= ftab[ar + q];
+            }
+            if (!m) continue;
+            // LDS bit of (k0 + r, j = 32 q + 1 + i) is b0 + i; the masks hold no j > J
+            const uint32_t b0 = (k0 + r - 1u) * J - (w0 << 5) + (q << 5);
+            const uint32_t sh = b0 & 31u, w = b0 >> 5;
+            atomicOr(&lds[w], m << sh);
+            if (sh) {
+                const uint32_t hi = m >> (32u - sh);
+                if (hi) atomicOr(&lds[w + 1u], hi);
+            }
+        }
+    }
 
     const uint32_t row = threadIdx.x % TD_ROWS;
     const uint32_t slot = threadIdx.x / TD_ROWS;
@@ -862,6 +990,7 @@ struct TdScratch {
     uint16_t *rdec = nullptr;    // [TD_RDEC_SIZE]
     uint32_t *items = nullptr;   // [TD_ITEM_CAP] row-kernel work items
     uint32_t *mid = nullptr;     // [TD_MID_CAP] compacted mid-range primes
+    uint32_t *fill = nullptr;    // [TD_FILL_WORDS] class masks of the fill primes
     hipStream_t aux = nullptr;   // row kernel, concurrent with the lane kernel
     hipEvent_t fork = nullptr, join = nullptr;
 };
@@ -876,6 +1005,7 @@ void td_scratch_release(TdScratch &s)
     if (s.rdec) (void)hipFree(s.rdec);
     if (s.items) (void)hipFree(s.items);
     if (s.mid) (void)hipFree(s.mid);
+    if (s.fill) (void)hipFree(s.fill);
     if (s.aux) (void)hipStreamDestroy(s.aux);
     if (s.fork) (void)hipEventDestroy(s.fork);
     if (s.join) (void)hipEventDestroy(s.join);
@@ -896,7 +1026,8 @@ int td_scratch(TdScratch **out)
                   hipMemset(s.ctrl, 0, CT_WORDS * sizeof(uint32_t)) == hipSuccess &&
                   hipMalloc(&s.rdec, TD_RDEC_SIZE * sizeof(uint16_t)) == hipSuccess &&
                   hipMalloc(&s.items, TD_ITEM_CAP * sizeof(uint32_t)) == hipSuccess &&
-                  hipMalloc(&s.mid, TD_MID_CAP * sizeof(uint32_t)) == hipSuccess;
+                  hipMalloc(&s.mid, TD_MID_CAP * sizeof(uint32_t)) == hipSuccess &&
+                  hipMalloc(&s.fill, TD_FILL_WORDS * sizeof(uint32_t)) == hipSuccess;
         if (ok) {
             td_rdec_kernel<<<(TD_RDEC_SIZE + 255u) / 256u, 256>>>(s.rdec);
             ok = hipGetLastError() == hipSuccess &&
@@ -935,14 +1066,18 @@ void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long
     // frees compute units, and the row kernel then ends after it: 16.7
     // against 14.9 ms per 8 triples of 4096 x 4096.)
     const hipStream_t rs = s.aux ? s.aux : st;
-    if (a - c >= DJ) td_items_kernel<DJ><<<1, 1024, 0, st>>>(s.tbl, s.ctrl, J, s.items);
+    if (a - c >= DJ) {
+        td_items_kernel<DJ><<<1, 1024, 0, st>>>(s.tbl, s.ctrl, J, s.items);
+        const uint32_t fw = 2u * TD_FILL_CLASSES * ((J + 31u) >> 5);
+        td_fill_table_kernel<DJ><<<(fw + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(J, s.fill);
+    }
     if (s.aux) {
         (void)hipEventRecord(s.fork, st);
         (void)hipStreamWaitEvent(s.aux, s.fork, 0);
     }
     if (a - c >= DJ)
         td_rows_bal_kernel<DJ><<<row_groups, TD_BLOCK, lds, rs>>>(
-            s.tbl, s.ctrl, s.items, s.rdec + TD_RDEC_OFF[DJ], K, J, d_bits);
+            s.tbl, s.ctrl, s.items, s.rdec + TD_RDEC_OFF[DJ], s.fill, K, J, d_bits);
     else
         td_rows_kernel<DJ><<<row_groups, TD_BLOCK, lds, rs>>>(s.tbl, s.ctrl, K, J, d_bits);
     // One merged rev residue per k needs every j digit group's exponent
@@ -986,7 +1121,8 @@ constexpr size_t TD_NA_CTRL  = TD_NA_TBL + (size_t)TD_SMALL_CAP * TD_TBL;
 constexpr size_t TD_NA_RDEC  = TD_NA_CTRL + 64;                      // 256-byte aligned
 constexpr size_t TD_NA_ITEMS = TD_NA_RDEC + ((TD_RDEC_SIZE * sizeof(uint16_t) + 255u) / 256u) * 64u;
 constexpr size_t TD_NA_MID   = TD_NA_ITEMS + TD_ITEM_CAP;
-constexpr size_t TD_NA_WORDS = TD_NA_MID + TD_MID_CAP;              // 11.37 MB
+constexpr size_t TD_NA_FILL  = TD_NA_MID + TD_MID_CAP;
+constexpr size_t TD_NA_WORDS = TD_NA_FILL + TD_FILL_WORDS;          // 11.75 MB
 static_assert(CT_WORDS <= 64, "control words fit their slot");
 
 static __device__ __align__(256) uint32_t scratch[TD_NA_WORDS];
@@ -1025,6 +1161,7 @@ int td_noalloc_scratch(hipStream_t st, TdScratch *out)
     out->rdec = (uint16_t *)(na.ptr + TD_NA_RDEC);
     out->items = na.ptr + TD_NA_ITEMS;
     out->mid = na.ptr + TD_NA_MID;
+    out->fill = na.ptr + TD_NA_FILL;
     return 0;
 }
 

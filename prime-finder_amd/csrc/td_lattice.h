@@ -102,10 +102,13 @@ PF_HD uint32_t td_rdec_arith(uint32_t R, int d)
 struct TdFh {
     uint32_t c;                 // the query's start value
     uint32_t L, A, M, tacc, w;
-    int lvl;
     bool neg, live, fail;       // live: levels still to run
     bool imm;                   // c < T: t = 0 is the hit
 };
+
+// A query needs at most 32 levels for p < 2^31; a composite list entry that
+// never decides is stopped after this many (wave-uniform loop bound).
+constexpr int TD_FH_MAX_LEVELS = 40;
 
 PF_HD void td_fh_start(TdFh &q, uint32_t c, uint32_t sig, uint32_t T, uint32_t p)
 {
@@ -115,7 +118,6 @@ PF_HD void td_fh_start(TdFh &q, uint32_t c, uint32_t sig, uint32_t T, uint32_t p
     q.M = p;
     q.tacc = 0u;
     q.w = 0u;
-    q.lvl = 0;
     q.neg = false;
     q.fail = false;
     q.imm = c < T;
@@ -123,35 +125,90 @@ PF_HD void td_fh_start(TdFh &q, uint32_t c, uint32_t sig, uint32_t T, uint32_t p
     if (q.imm) TD_LAT_COUNT(query);
 }
 
-// One level of the descent; clears q.live when the query is decided.
-PF_HD void td_fh_level(TdFh &q, uint32_t T)
+// 1/A (A >= 1) scaled by 1 - 2^-20, so that trunc(x * td_rcp_lo(A)) never
+// exceeds floor(x / A) for x < 2^31: the float rounding of x, of A and of
+// the reciprocal (v_rcp_f32: 1 ulp) and of the product add up to less than
+// 5 * 2^-24 relative, which the 2^-20 bias covers.
+PF_HD float td_rcp_lo(uint32_t A)
 {
-    TD_LAT_COUNT(level);
-    const bool mir = q.A > q.M - q.A;    // mirror: (M - A) x mod M in [M - R, M - L]
-    const uint32_t R = q.L + T - 1u;
-    const uint32_t A = mir ? q.M - q.A : q.A;
-    const uint32_t L = mir ? q.M - R : q.L;
-    q.tacc += mir ? (q.neg ? 0u - q.M : q.M) : 0u;
-    q.neg = q.neg != mir;
-    const uint32_t Ad = A ? A : 1u;
-    const uint32_t qt = L / Ad;
-    const uint32_t rL = L - qt * Ad;
-    const uint32_t up = rL ? Ad - rL : 0u;       // distance to the next multiple of A
-    const uint32_t mA = q.M % Ad;
-    const bool done = up < T;                    // a multiple lands in [L, R]
-    const bool fail = A == 0u || (!done && mA == 0u) || q.lvl >= 40;
-    if (done || fail) {
-        TD_LAT_COUNT(query);
-        q.live = false;
-        q.fail = fail;
-        q.w = L + up;
-        return;
+#if defined(__HIP_DEVICE_COMPILE__)
+    const float r = __builtin_amdgcn_rcpf((float)A);
+#else
+    const float r = 1.0f / (float)A;
+#endif
+    return r * 0.99999904632568359375f;          // 1 - 2^-20, exact in binary32
+}
+
+// x mod A from the biased reciprocal: the estimate q of floor(x / A) is that
+// quotient or one less while the quotient is below 2^19, which the caller
+// guarantees by checking est < 2^18 (an estimate below 2^18 implies a
+// quotient below 2^19); then x - q A is in [0, 2 A) and one conditional
+// subtraction (as a minimum of unsigned values) finishes it.
+PF_HD uint32_t td_mod_est(uint32_t x, uint32_t A, float rA, uint32_t &est)
+{
+    est = (uint32_t)((float)x * rA);
+    const uint32_t r = x - est * A;
+    const uint32_t d = r - A;
+    return d < r ? d : r;
+}
+
+// The levels of a started query, until it is decided (or failed after
+// TD_FH_MAX_LEVELS: the bound is the same in every lane, so the loop
+// counter is wave-uniform).  Per level, with A > M/2 first mirrored:
+// L mod A and M mod A from one biased reciprocal (exact division only for
+// a partial quotient of 2^18 or more, which is rare), then either the next
+// multiple of A lands in [L, L + T - 1] (decided) or the level moves to
+// (A - M mod A, A, L mod A).  (M mod A == 0 makes the next level's A
+// vanish, which fails there.)
+PF_HD void td_fh_run(TdFh &q, uint32_t T)
+{
+    if (!q.live) return;
+    uint32_t L0 = q.L, A0 = q.A, M = q.M, tacc = q.tacc, w = 0u;
+    bool neg = q.neg, live = true, fail = false;
+    for (int it = 0; it < TD_FH_MAX_LEVELS; it++) {
+        TD_LAT_COUNT(level);
+        const uint32_t MA = M - A0;
+        const bool mir = A0 > MA;        // mirror: (M - A) x mod M in [M - R, M - L]
+        const uint32_t A = mir ? MA : A0;
+        const uint32_t L = mir ? M - (L0 + (T - 1u)) : L0;
+        const uint32_t sM = neg ? 0u - M : M;
+        tacc += mir ? sM : 0u;
+        neg = neg != mir;
+        const uint32_t Ad = A ? A : 1u;
+        const float rA = td_rcp_lo(Ad);
+        uint32_t eL, eM;
+        uint32_t rL = td_mod_est(L, Ad, rA, eL);
+        uint32_t mA = td_mod_est(M, Ad, rA, eM);
+        const bool slow = (eL | eM) >= (1u << 18);
+#if defined(__HIP_DEVICE_COMPILE__)
+        if (__ballot(slow) != 0ull) {
+#else
+        {
+#endif
+            if (slow) {
+                rL = L % Ad;
+                mA = M % Ad;
+            }
+        }
+        const uint32_t up = rL ? Ad - rL : 0u;   // distance to the next multiple of A
+        if (up < T || A == 0u) {                 // decided: a multiple lands in [L, R]
+            TD_LAT_COUNT(query);
+            live = false;
+            fail = A == 0u;
+            w = L + up;
+            break;
+        }
+        tacc += neg ? rL - L : L - rL;
+        M = A;
+        A0 = A - mA;
+        L0 = rL;
     }
-    q.tacc += q.neg ? rL - L : L - rL;
-    q.M = A;
-    q.A = A - mA;
-    q.L = rL;
-    q.lvl++;
+    if (live) TD_LAT_COUNT(query);
+    q.tacc = tacc;
+    q.neg = neg;
+    q.w = w;
+    q.fail = fail || live;
+    q.live = false;
 }
 
 // The decided query's t (0xFFFFFFFF: none) and residue y.
@@ -178,7 +235,7 @@ PF_HD uint32_t td_first_hit(uint32_t c, uint32_t sig, uint32_t T, uint32_t sigin
 {
     TdFh q;
     td_fh_start(q, c, sig, T, m.p);
-    while (q.live) td_fh_level(q, T);
+    td_fh_run(q, T);
     return td_fh_result(q, siginvM, m, y);
 }
 
@@ -410,6 +467,11 @@ PF_HD void td_lattice_prime(uint32_t p, long long a, long long b, long long c, u
 {
     TdLatRec r;
     td_lattice_setup<DJ>(p, a, b, c, K, J, mark, r);
+    // (unrolled, so that the record's arrays are indexed by constants and
+    // stay in registers)
+#if defined(__HIPCC__)
+#pragma unroll
+#endif
     for (int cond = 0; cond < TD_LAT_CONDS; cond++) {
         TdLatItem it;
         if (td_lattice_item(r, cond, K, J, td_pow10(DJ) - 1u, it))
