@@ -48,11 +48,15 @@ DJ = 4
 #   gapstep one return-time step from a hit to the next (two compares, the
 #           selects of the step and of the landing residue, two adds, the
 #           end test): 8;
+#   gapcf   one division step of the return-time computation (a run of the
+#           slow continued fraction: the run length and the first value
+#           below the window, two divisions by the same divisor at 10 each,
+#           and 18 compare/select/multiply-add ops): 38;
 #   kstep   one k of the split k loop (primes from 2^16 below the lattice
 #           range, beside the lattice kernel on the second stream): two
 #           residues advanced and compared, 8.
 TD_OPS_PER_UNIT = {"modmul": 8, "level": 45, "query": 10, "hit": 12, "step": 4, "gapstep": 8,
-                   "kstep": 8}
+                   "gapcf": 38, "kstep": 8}
 INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12      # 39.32 Tops/s (256 CUs, wave64, 2.4 GHz)
 # HBM bytes per launch of the lattice kernel on this workload, measured with
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 256 MiB copy kernel's

@@ -141,6 +141,7 @@ enum {
     TD_WORK_STEP,        /* direct steps of the one-digit rev class         */
     TD_WORK_KSTEP,       /* prime x k steps of the k-loop kernels (>= 2^16) */
     TD_WORK_GAPSTEP,     /* return-time steps from hit to hit               */
+    TD_WORK_GAPCF,       /* division steps computing those return times     */
     TD_WORK_FIELDS
 };
 int trial_div_work(const uint32_t *primes, uint32_t n_primes,

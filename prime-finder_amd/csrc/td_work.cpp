@@ -2,8 +2,9 @@
 // on the host, for rating the kernels against the INT32 roof.  The lattice
 // path is counted by running its per-prime routine (td_lattice.h, the code
 // the lattice kernel runs) with counting hooks: Montgomery multiplications,
-// first-hit queries, continued-fraction levels, hits, the return-time steps
-// between hits and the direct steps of the one-digit rev class.  The k-loop paths (primes from 2^16 below the
+// first-hit queries, continued-fraction levels, hits, the division steps that
+// compute the return times, the return-time steps between hits and the
+// direct steps of the one-digit rev class.  The k-loop paths (primes from 2^16 below the
 // lattice range, or every prime from 2^16 when the j digit groups do not
 // merge) are counted as prime x k steps.  No GPU call.
 #include <stdint.h>
@@ -14,7 +15,7 @@
 
 namespace {
 struct TdWork {
-    uint64_t prime, modmul, query, level, hit, step, gapstep;
+    uint64_t prime, modmul, query, level, hit, step, gapstep, gapcf;
 };
 thread_local TdWork *td_work_cur;
 }  // namespace
@@ -62,7 +63,7 @@ extern "C" int trial_div_work(const uint32_t *primes, uint32_t n_primes, long lo
     out[TD_WORK_KSTEP] = ksteps;
     if (!merged) return 0;
     const unsigned nt = threads ? threads : 1u;
-    std::vector<TdWork> part(nt, TdWork{0, 0, 0, 0, 0, 0, 0});
+    std::vector<TdWork> part(nt, TdWork{0, 0, 0, 0, 0, 0, 0, 0});
     std::vector<std::thread> pool;
     for (unsigned t = 0; t < nt; t++)
         pool.emplace_back([&, t] {
@@ -82,6 +83,7 @@ extern "C" int trial_div_work(const uint32_t *primes, uint32_t n_primes, long lo
         out[TD_WORK_HIT] += w.hit;
         out[TD_WORK_STEP] += w.step;
         out[TD_WORK_GAPSTEP] += w.gapstep;
+        out[TD_WORK_GAPCF] += w.gapcf;
     }
     return 0;
 }

@@ -141,7 +141,8 @@ def trial_div_gpu_kj(primes: np.ndarray, a: int, b: int, c: int, K: int, J: int,
     return rc, out
 
 
-TD_WORK_FIELDS = ("prime", "modmul", "query", "level", "hit", "step", "kstep", "gapstep")
+TD_WORK_FIELDS = ("prime", "modmul", "query", "level", "hit", "step", "kstep", "gapstep",
+                  "gapcf")
 
 
 def trial_div_work(primes: np.ndarray, a: int, b: int, c: int, K: int, J: int,

@@ -38,10 +38,8 @@ extern "C" int lattice_sieve(const uint32_t *primes, uint32_t n, long long a, lo
 // v where the rule and the brute force differ (-1: td_gaps failed).
 extern "C" int gaps_rule_mismatches(uint32_t p, uint32_t sig, uint32_t T)
 {
-    const PfMont m = pf_mont_init(p);
-    const uint32_t siginvM = pf_mont_pow(pf_mont_in(sig, m), (uint64_t)p - 2u, m);
     TdGaps g;
-    if (!td_gaps(sig, T, siginvM, m, g)) return -1;
+    if (!td_gaps(sig, T, p, g)) return -1;
     if ((uint64_t)g.alpha + g.beta < T) return -2;
     int bad = 0;
     for (uint32_t v = 0; v < T; v++) {
