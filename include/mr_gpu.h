@@ -185,6 +185,20 @@ int64_t mr_sched_job_bytes(int64_t bits);
 /* Queue n candidates (validated first; -1 and nothing queued if any cannot
  * be formed or is too large). */
 int mr_sched_submit(mr_sched_t *s, const mr_candidate_t *cands, const int64_t *tags, int n);
+/* Queue the survivors of one trial-division call in place: d_list is a
+ * device list as trial_div_survivors writes it ((k << 16) | j per entry),
+ * n how many of its entries to test (e.g. the count trial_div_survivors
+ * returned), each as the candidate (a, b, c, k, j, which) with tag
+ * tag_base + i.  The (k, j) values stay on the device: a job's number is
+ * built from its list entry when a slot takes it.  The list must stay
+ * valid and unchanged until every job of the call is decided.  When
+ * max(b, c) > a - 7 the decimal length may vary between survivors; the list
+ * is then read to the host and queued as mr_sched_submit would.  Returns
+ * 0, -1 for invalid arguments or candidates (nothing queued), -2 on a HIP
+ * error.  No counterpart in the reference, whose worker sends survivors to
+ * its master over MPI (worker.c:45-54, master.c:518-531). */
+int mr_sched_submit_survivors(mr_sched_t *s, long long a, long long b, long long c,
+                              const uint32_t *d_list, uint32_t n, int which, int64_t tag_base);
 /* Advance the scheduler.  block = 0: launch what can be launched and
  * return; block = 1: also wait for the next chunk or admission batch to
  * finish.  Returns the number of candidates not yet decided (>= 0), or -2

@@ -1807,83 +1807,145 @@ extern "C" int mr_ctx_from_residue_dev(mr_ctx_t *x, const uint64_t *d_x, int ld,
 
 namespace {
 
-int ndig_u(uint64_t v)
+__host__ __device__ int ndig_u(uint64_t v)
 {
     int n = 1;
     while (v >= 10u) { v /= 10u; n++; }
     return n;
 }
 
-uint64_t rev_u(uint64_t v, int len)
+__host__ __device__ uint64_t rev_u(uint64_t v, int len)
 {
     uint64_t r = 0;
     for (int i = 0; i < len; i++) { r = r * 10u + v % 10u; v /= 10u; }
     return r;
 }
 
-// value = sum coef * 10^exp (exponents distinct), and the decimal length
-// bound of the value.  Returns false for an invalid candidate.
-bool candidate_terms(const mr_candidate_t &cd, std::vector<std::pair<int64_t, int64_t>> &terms,
-                     int64_t *digits)
+// value = sum coef * 10^exp (exponents distinct) as TMAX (coef, exp) pairs
+// in terms[2 TMAX] (unused pairs zero), and the decimal length of the value
+// in *digits.  False for an invalid candidate.  The same code builds the
+// terms on the host (candidates submitted from the host) and on the device
+// (survivor lists submitted in place, surv_terms_kernel).
+__host__ __device__ bool cand_terms(int64_t a, int64_t b, int64_t c, uint64_t k, uint64_t j,
+                                    int which, int64_t *terms, int64_t *digits)
 {
-    const int64_t a = cd.a, b = cd.b, c = cd.c;
-    const uint64_t k = cd.k, j = cd.j;
     if (a < 1 || b < 0 || c < 0 || k < 1 || j < 1 || k > 1000000u || j > 1000000u) return false;
-    std::vector<std::pair<int64_t, int64_t>> raw;
-    if (cd.which == 2) {
+    int64_t raw[2 * TMAX];
+    int nr = 0;
+    if (which == 2) {
         const int lk = ndig_u(k), lj = ndig_u(j);
         const int64_t eb = a - b - lk, ec = a - c - lj;
         if (eb < 0 || ec < 0 || a >= 2000000) return false;
-        raw = {{1, a}, {-(int64_t)rev_u(k, lk), eb}, {-(int64_t)rev_u(j, lj), ec}, {-1, 0}};
+        const int64_t r4[8] = {1, a, -(int64_t)rev_u(k, lk), eb, -(int64_t)rev_u(j, lj), ec, -1, 0};
+        for (int i = 0; i < 8; i++) raw[i] = r4[i];
+        nr = 4;
         *digits = a;
-    } else if (cd.which == 0 || cd.which == 1) {
+    } else if (which == 0 || which == 1) {
         // S = k 10^b + j 10^c as sparse decimal digits; N = (10^a - 1) - S
         // has digits 9 - s_i with no borrow, when S < 10^a - 1.
-        std::vector<std::pair<int64_t, int>> sd;   // (position, digit), s_i != 0
+        int64_t sp[20];                 // positions of S's nonzero digits
+        int sd[20];                     // and the digits (at most 19: S < 10^19 below)
+        int ns = 0;
         const int64_t lo = b < c ? b : c, hi = b < c ? c : b;
         const uint64_t clo = b < c ? k : j, chi = b < c ? j : k;
         if (hi - lo <= 12) {
             uint64_t x = chi;
             for (int64_t i = 0; i < hi - lo; i++) x *= 10u;
             x += clo;
-            for (int64_t p = lo; x; p++, x /= 10u) if (x % 10u) sd.push_back({p, (int)(x % 10u)});
+            for (int64_t q = lo; x; q++, x /= 10u)
+                if (x % 10u) { sp[ns] = q; sd[ns] = (int)(x % 10u); ns++; }
         } else {
             uint64_t x = clo;
-            for (int64_t p = lo; x; p++, x /= 10u) if (x % 10u) sd.push_back({p, (int)(x % 10u)});
+            for (int64_t q = lo; x; q++, x /= 10u)
+                if (x % 10u) { sp[ns] = q; sd[ns] = (int)(x % 10u); ns++; }
             x = chi;
-            for (int64_t p = hi; x; p++, x /= 10u) if (x % 10u) sd.push_back({p, (int)(x % 10u)});
+            for (int64_t q = hi; x; q++, x /= 10u)
+                if (x % 10u) { sp[ns] = q; sd[ns] = (int)(x % 10u); ns++; }
         }
-        for (auto &q : sd) if (q.first >= a) return false;          // S >= 10^a
+        for (int i = 0; i < ns; i++) if (sp[i] >= a) return false;          // S >= 10^a
         // decimal length D of N: drop leading digits 9 - 9 = 0
         int64_t D = a;
         for (;;) {
             if (D == 0) return false;
             bool nine = false;
-            for (auto &q : sd) if (q.first == D - 1 && q.second == 9) nine = true;
+            for (int i = 0; i < ns; i++) if (sp[i] == D - 1 && sd[i] == 9) nine = true;
             if (!nine) break;
             D--;
         }
         if (a >= 2000000) return false;
-        if (cd.which == 0) {
-            raw = {{1, a}, {-1, 0}};
-            for (auto &q : sd) raw.push_back({-(int64_t)q.second, q.first});
-            *digits = a;
-        } else {
-            raw = {{1, D}, {-1, 0}};
-            for (auto &q : sd)
-                if (q.first < D) raw.push_back({-(int64_t)q.second, D - 1 - q.first});
-            *digits = D;
+        raw[0] = 1;
+        raw[1] = which == 0 ? a : D;
+        raw[2] = -1;
+        raw[3] = 0;
+        nr = 2;
+        for (int i = 0; i < ns; i++) {
+            if (which == 0) {
+                raw[2 * nr] = -(int64_t)sd[i];
+                raw[2 * nr + 1] = sp[i];
+                nr++;
+            } else if (sp[i] < D) {
+                raw[2 * nr] = -(int64_t)sd[i];
+                raw[2 * nr + 1] = D - 1 - sp[i];
+                nr++;
+            }
         }
+        *digits = which == 0 ? a : D;
     } else {
         return false;
     }
-    terms.clear();
-    for (auto &r : raw) {
+    int nt = 0;
+    for (int r = 0; r < nr; r++) {
         bool merged = false;
-        for (auto &t : terms)
-            if (t.second == r.second) { t.first += r.first; merged = true; }
-        if (!merged) terms.push_back(r);
+        for (int t = 0; t < nt; t++)
+            if (terms[2 * t + 1] == raw[2 * r + 1]) { terms[2 * t] += raw[2 * r]; merged = true; }
+        if (merged) continue;
+        if (nt == TMAX) return false;
+        terms[2 * nt] = raw[2 * r];
+        terms[2 * nt + 1] = raw[2 * r + 1];
+        nt++;
     }
+    for (int t = nt; t < TMAX; t++) terms[2 * t] = terms[2 * t + 1] = 0;
+    return true;
+}
+
+// Where an admitted job's terms come from: the host (list == nullptr, terms
+// copied with the admission) or entry idx of a device survivor list.
+struct SurvJob {
+    const uint32_t *list;
+    int64_t a, b, c, digits;
+    uint32_t idx;
+    int32_t which;
+};
+
+// Terms of admitted jobs whose (k, j) is an entry of a device survivor list
+// ((k << 16) | j, as trial_div_survivors writes it): one thread per job, the
+// row of the others (host-built, list == nullptr) left as copied.  A
+// candidate that is invalid or has another decimal length than its slot
+// was sized for sets bit 2 of the error latch.
+__global__ void surv_terms_kernel(const SurvJob *jobs, int n, int64_t *terms, uint32_t *err)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SurvJob jb = jobs[i];
+    if (!jb.list) return;
+    const uint32_t v = jb.list[jb.idx];
+    int64_t *t = terms + (size_t)i * TMAX * 2;
+    int64_t dg = 0;
+    if (!cand_terms(jb.a, jb.b, jb.c, v >> 16, v & 0xFFFFu, jb.which, t, &dg) || dg != jb.digits) {
+        for (int q = 0; q < 2 * TMAX; q++) t[q] = 0;
+        atomicOr(err, 2u);
+    }
+}
+
+// cand_terms for a host candidate, as (coef, exp) pairs.
+bool candidate_terms(const mr_candidate_t &cd, std::vector<std::pair<int64_t, int64_t>> &terms,
+                     int64_t *digits)
+{
+    int64_t t[2 * TMAX];
+    if (!cand_terms(cd.a, cd.b, cd.c, cd.k, cd.j, cd.which, t, digits)) return false;
+    terms.clear();
+    for (int i = 0; i < TMAX; i++)
+        if (t[2 * i] != 0 || t[2 * i + 1] != 0) terms.push_back({t[2 * i], t[2 * i + 1]});
     return true;
 }
 
@@ -1993,13 +2055,20 @@ struct Req {
     int64_t tag;
     double t_submit, t_admit;
     int L, logm;
+    int64_t digits = 0;             // decimal length (sets L and logm)
     int rounds;
     int64_t terms[TMAX * 2];
     // restored from a checkpoint after admission: the base in progress, the
     // next exponent bit and the residue (L limbs) to resume from
     int resume_bidx = -1, resume_bitpos = 0;
     std::vector<uint32_t> resume_res;
+    // submitted in place from a device survivor list (mr_sched_submit_survivors):
+    // (k, j) is entry dev_idx of dev_list, (k << 16) | j, read by the device
+    // when the job is admitted; cand.k = cand.j = 0 until then
+    const uint32_t *dev_list = nullptr;
+    uint32_t dev_idx = 0;
 };
+
 
 // Bytes of device memory per slot of a pool of transform length 2^logm
 // (the arrays of ctx_alloc_all).
@@ -2029,6 +2098,7 @@ struct Pool {
     int32_t *h_list_adm = nullptr, *h_Ls = nullptr;
     int32_t *h_bitpos = nullptr, *h_sval = nullptr;
     int64_t *h_terms = nullptr;
+    SurvJob *h_sjobs = nullptr, *d_sjobs = nullptr;   // [n] terms sources of an admission
     uint8_t *h_alive = nullptr, *h_done = nullptr;
     uint32_t *h_err = nullptr;
     std::vector<uint8_t> state;
@@ -2080,8 +2150,8 @@ void pool_free(Pool *p)
     if (p->ev_run) (void)hipEventDestroy(p->ev_run);
     if (p->ev_adm) (void)hipEventDestroy(p->ev_adm);
     if (p->st_run) (void)hipStreamDestroy(p->st_run);
-    void *hp[] = {p->h_list_adm, p->h_Ls, p->h_bitpos, p->h_sval,
-                  p->h_terms, p->h_alive, p->h_done, p->h_err, p->h_resume, p->h_resres, p->h_run};
+    void *hp[] = {p->h_list_adm, p->h_Ls, p->h_bitpos, p->h_sval, p->h_terms, p->h_sjobs,
+                  p->h_alive, p->h_done, p->h_err, p->h_resume, p->h_resres, p->h_run};
     for (void *q : hp) if (q) (void)hipHostFree(q);
     if (p->x) mr_ctx_destroy(p->x);       // frees the device arrays it owns
     delete p;
@@ -2110,6 +2180,7 @@ int pool_new(mr_sched *S, int logm, Pool **out)
     dev((void **)&p->d_list_adm, 4 * (size_t)n);
     dev((void **)&p->d_Ls, 4 * (size_t)n);
     dev((void **)&p->d_terms, sizeof(int64_t) * TMAX * 2 * (size_t)n);
+    dev((void **)&p->d_sjobs, sizeof(SurvJob) * (size_t)n);
     dev((void **)&p->d_bases, 4 * (size_t)S->n_bases);
     dev((void **)&p->d_resume, 12 * (size_t)n);
     dev((void **)&p->d_run, (size_t)n);
@@ -2121,6 +2192,7 @@ int pool_new(mr_sched *S, int logm, Pool **out)
     host((void **)&p->h_bitpos, 4 * (size_t)n);
     host((void **)&p->h_sval, 4 * (size_t)n);
     host((void **)&p->h_terms, sizeof(int64_t) * TMAX * 2 * (size_t)n);
+    host((void **)&p->h_sjobs, sizeof(SurvJob) * (size_t)n);
     host((void **)&p->h_alive, (size_t)n);
     host((void **)&p->h_done, (size_t)n);
     host((void **)&p->h_err, 4);
@@ -2262,6 +2334,7 @@ int launch_admin(mr_sched *S, Pool *p)
         if (p->state[c] == SL_END) p->adm_list.push_back(c);
     const int n_end = (int)p->adm_list.size();
     int n_res = 0;          // restored candidates among the admitted
+    int n_dev = 0;          // admitted from device survivor lists
     std::vector<int> &queue = S->queue[p->logm];
     size_t &qh = S->qhead[p->logm];
     const int64_t per = slot_bytes(p->logm);
@@ -2274,6 +2347,9 @@ int launch_admin(mr_sched *S, Pool *p)
         Req &q = S->reqs[r];
         const int i = (int)p->adm_list.size() - n_end;
         memcpy(p->h_terms + (size_t)i * TMAX * 2, q.terms, sizeof(q.terms));
+        p->h_sjobs[i] = SurvJob{q.dev_list, q.cand.a, q.cand.b, q.cand.c, q.digits, q.dev_idx,
+                                q.cand.which};
+        n_dev += q.dev_list ? 1 : 0;
         p->h_Ls[i] = q.L;
         if (q.resume_bidx >= 0) {
             if (!p->h_resres &&
@@ -2307,6 +2383,13 @@ int launch_admin(mr_sched *S, Pool *p)
         MR_CHECK(hipMemcpyAsync(p->d_terms, p->h_terms, sizeof(int64_t) * TMAX * 2 * (size_t)n_new,
                                 hipMemcpyHostToDevice, st));
         MR_CHECK(hipMemcpyAsync(p->d_Ls, p->h_Ls, 4 * (size_t)n_new, hipMemcpyHostToDevice, st));
+        if (n_dev) {        // terms of jobs from device survivor lists, built in place
+            MR_CHECK(hipMemcpyAsync(p->d_sjobs, p->h_sjobs, sizeof(SurvJob) * (size_t)n_new,
+                                    hipMemcpyHostToDevice, st));
+            surv_terms_kernel<<<(n_new + 63) / 64, 64, 0, st>>>(p->d_sjobs, n_new, p->d_terms,
+                                                                 p->x->d.err);
+            MR_CHECK(hipGetLastError());
+        }
     }
     Dev De = p->x->d, Dn = p->x->d;
     De.list = p->d_list_adm;
@@ -2356,7 +2439,8 @@ int admin_done(mr_sched *S, Pool *p)
 {
     p->adm_busy = false;
     if (*p->h_err) {
-        fprintf(stderr, "%s: a candidate does not fit its slot\n", kWho);
+        fprintf(stderr, "%s: a candidate does not fit its slot (or a survivor-list entry is "
+                        "not a valid (k, j))\n", kWho);
         return -2;
     }
     for (int i = 0; i < (int)p->adm_list.size(); i++) {
@@ -2424,6 +2508,7 @@ bool make_req(const mr_candidate_t &cd, int64_t tag, Req &q)
         return false;
     }
     geometry(digits_to_bits(dg), MIN_LOGM, &q.L, &q.logm);
+    q.digits = dg;
     if (q.logm > MAX_LOGM || q.L > MAX_LIMBS) {
         fprintf(stderr, "%s: candidate of %lld digits is beyond this build's transform sizes\n", kWho,
                 (long long)dg);
@@ -2474,6 +2559,42 @@ extern "C" int mr_sched_submit(mr_sched_t *S, const mr_candidate_t *cands, const
     return enqueue(S, add);
 }
 
+extern "C" int mr_sched_submit_survivors(mr_sched_t *S, long long a, long long b, long long c,
+                                         const uint32_t *d_list, uint32_t n, int which,
+                                         int64_t tag_base)
+{
+    if (!S || (!d_list && n) || which < 0 || which > 2 || a < 1 || b < 0 || c < 0) return -1;
+    if (n == 0) return 0;
+    if (hipSetDevice(S->device) != hipSuccess) return -2;
+    std::vector<Req> add((size_t)n);
+    if (std::max(b, c) > a - 7) {
+        // k 10^b + j 10^c may reach the leading digit, so the decimal length
+        // (and the slot size) can differ between survivors: the host path
+        std::vector<uint32_t> h(n);
+        const hipError_t e = hipMemcpy(h.data(), d_list, 4 * (size_t)n, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return hip_fail(e, "survivor list");
+        for (uint32_t i = 0; i < n; i++) {
+            mr_candidate_t cd{a, b, c, h[i] >> 16, h[i] & 0xFFFFu, which};
+            if (!make_req(cd, tag_base + i, add[i])) return -1;
+        }
+        return enqueue(S, add);
+    }
+    // k, j < 2^16 < 10^5, so k 10^b + j 10^c < 10^(a-1): every survivor has a
+    // decimal digits, and one representative fixes the slot size
+    Req proto;
+    if (!make_req(mr_candidate_t{a, b, c, 1u, 1u, which}, 0, proto)) return -1;
+    for (uint32_t i = 0; i < n; i++) {
+        Req &q = add[i];
+        q = proto;
+        memset(q.terms, 0, sizeof(q.terms));
+        q.cand.k = q.cand.j = 0u;
+        q.tag = tag_base + i;
+        q.dev_list = d_list;
+        q.dev_idx = i;
+    }
+    return enqueue(S, add);
+}
+
 namespace {
 
 int sched_checkpoint(mr_sched *S, const char *path)
@@ -2515,6 +2636,15 @@ int sched_checkpoint(mr_sched *S, const char *path)
         for (size_t i = S->qhead[lg]; i < S->queue[lg].size(); i++)
             rows.push_back(Row{S->queue[lg][i], -1, 0, {}});
     std::sort(rows.begin(), rows.end(), [](const Row &x, const Row &y) { return x.r < y.r; });
+    // jobs submitted from device survivor lists: read their (k, j) now
+    for (const Row &w : rows) {
+        Req &q = S->reqs[w.r];
+        if (!q.dev_list || q.cand.k) continue;
+        uint32_t v = 0;
+        MR_CHECK(hipMemcpy(&v, q.dev_list + q.dev_idx, 4, hipMemcpyDeviceToHost));
+        q.cand.k = v >> 16;
+        q.cand.j = v & 0xFFFFu;
+    }
     // 3. the file (written whole, then renamed over path)
     const std::string tmp = std::string(path) + ".tmp";
     FILE *f = fopen(tmp.c_str(), "w");

@@ -262,6 +262,8 @@ def _mr_sigs(L):
         "mr_sched_create": (vp, [C.POINTER(u32), i32, i32, i32, i32, C.c_int64]),
         "mr_sched_job_bytes": (C.c_int64, [C.c_int64]),
         "mr_sched_submit": (i32, [vp, C.POINTER(MrCandidate), C.POINTER(C.c_int64), i32]),
+        "mr_sched_submit_survivors": (i32, [vp, C.c_longlong, C.c_longlong, C.c_longlong, vp,
+                                            u32, i32, C.c_int64]),
         "mr_sched_pump": (i32, [vp, i32]),
         "mr_sched_results": (i32, [vp, C.POINTER(MrResult), i32]),
         "mr_sched_stats": (i32, [vp, C.POINTER(MrSchedStats)]),
@@ -369,6 +371,16 @@ class MrScheduler:
             t = (C.c_int64 * len(cands))(*tags)
         if self.L.mr_sched_submit(self.ptr, arr, t, len(cands)) != 0:
             raise ValueError("mr_sched_submit rejected the candidates")
+
+    def submit_survivors(self, a, b, c, d_list, n, which=0, tag_base=0):
+        """Queue the first n entries of a device survivor list ((k << 16) | j
+        per entry, a device pointer or DeviceBuffer) as candidates of the
+        triple (a, b, c), with tags tag_base + i; the list must outlive
+        the jobs."""
+        ptr = d_list.ptr if isinstance(d_list, DeviceBuffer) else d_list
+        rc = self.L.mr_sched_submit_survivors(self.ptr, a, b, c, ptr, n, which, tag_base)
+        if rc != 0:
+            raise ValueError(f"mr_sched_submit_survivors rejected the list ({rc})")
 
     def pump(self, block: bool = True) -> int:
         left = self.L.mr_sched_pump(self.ptr, 1 if block else 0)
