@@ -30,6 +30,15 @@ PF_HD uint32_t pf_submod(uint32_t x, uint32_t y, uint32_t p)
     return d < e ? d : e;
 }
 
+// (x + y) mod p for x, y in [0, p), p <= 2^31: the sum fits 32 bits, and the
+// minimum of it and its difference with p (which wraps above 2^31 when the
+// sum is below p) is the reduced value.  3 VALU ops, no branch.
+PF_HD uint32_t pf_addmod(uint32_t x, uint32_t y, uint32_t p)
+{
+    const uint32_t s = x + y, d = s - p;
+    return d < s ? d : s;
+}
+
 // Montgomery arithmetic with R = 2^32 for an odd modulus p < 2^31.
 struct PfMont {
     uint32_t p;

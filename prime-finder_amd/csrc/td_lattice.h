@@ -384,12 +384,21 @@ template <int DJ, class Mark>
 PF_HD void td_lattice_setup(uint32_t p, long long a, long long b, long long c, uint32_t K,
                             uint32_t J, Mark &mark, TdLatRec &r)
 {
-    const PfMont m = pf_mont_init(p);
-    const uint32_t ten = pf_mont_in(10u, m);
-    // 10^{-1}: 10 x = 1 + t p with t p == 9 (mod 10), t fixed by p mod 10.
+    const PfMont m = pf_mont_init(p);          // (its R^2 mod p is not needed here)
+    // 10 and 10^-1 in Montgomery form without R^2 mod p: 10 R by doubling
+    // R mod p, and R / 10 = (R + k p) / 10 with k < 10 the multiple of p
+    // that makes R + k p divisible by 10 (k = -R p^-1 mod 10; p is prime to
+    // 10), divided exactly: halve, then multiply by 5^-1 mod 2^32 (the
+    // quotient is below p < 2^31).
+    const uint32_t two = pf_addmod(m.one, m.one, p), four = pf_addmod(two, two, p);
+    const uint32_t ten = pf_addmod(pf_addmod(four, four, p), two, p);
     const uint32_t pm = p % 10u;
-    const uint32_t t = pm == 1u ? 9u : pm == 3u ? 3u : pm == 7u ? 7u : 1u;
-    const uint32_t tenInv = pf_mont_in((uint32_t)((1ull + (uint64_t)t * p) / 10u), m);
+    const uint32_t pinv10 = pm == 3u ? 7u : pm == 7u ? 3u : pm;          // p^-1 mod 10
+    const uint32_t r10 = m.one - 10u * (uint32_t)(((uint64_t)m.one * 0xCCCCCCCDull) >> 35);
+    const uint32_t kx = (r10 ? 10u - r10 : 0u) * pinv10;                  // < 90
+    const uint32_t k10 = kx - 10u * ((kx * 52429u) >> 19);
+    const uint64_t X = (uint64_t)m.one + (uint64_t)k10 * p;               // 10 | X, X < 10 p
+    const uint32_t tenInv = (uint32_t)(X >> 1) * 0xCCCCCCCDu;
 
     // Montgomery-form powers (suffix M) and plain residues; the powers of
     // 10 and of 10^-1 share their squarings.
