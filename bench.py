@@ -693,7 +693,8 @@ def main() -> int:
                      "traffic": TD_TRAFFIC_BYTES,
                      "traffic_note": "HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE); "
                                      "algorithmic 25.1e6 (prime list + bitmap), the rest are "
-                                     "hit atomics; 0.25 TB/s, far below the 8 TB/s roof",
+                                     f"hit atomics; {TD_TRAFFIC_BYTES / avg_lane_s / 1e12:.2f} "
+                                     "TB/s, far below the 8 TB/s roof",
                      "kernel": "td_lattice_kernel<4>",
                      "kernel_avg_ms": avg_lane_s * 1e3,
                      "ops_per_launch": alg_ops,
