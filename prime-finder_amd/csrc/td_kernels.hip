@@ -111,7 +111,7 @@ constexpr uint32_t TD_MID_CAP = 1100000;
 // marking steps, so that lanes of one wave get comparable work.
 constexpr uint32_t TD_CH       = 32;
 constexpr uint32_t TD_ITEM_CAP = 32768;   // > 2 * 6542 + 20000/32 * sum(1/p)
-enum { TI_N = 0, TI_REV = 1, TI_WHOLE = 2 };
+enum { TI_N = 0, TI_REV = 1, TI_WHOLE = 2, TI_ONE = 3 };
 
 __device__ __forceinline__ int td_ndigits(uint32_t x)
 {
@@ -309,8 +309,10 @@ void td_rows_kernel(const uint32_t *__restrict__ tbl, const uint32_t *__restrict
 // ceil(ceil(J/p) / TD_CH) normal-solve parts when the solve is active and
 // ceil(ceil((10^DJ - 1)/p) / TD_CH) merged-rev parts; p = 2 and 5 (whose
 // powers of ten vanish, so digit groups are gated one by one) keep one
-// whole-prime item, and the fill primes none (the row kernel ORs their
-// class masks).  Item = index | kind << 16 | part << 18.
+// whole-prime item, the fill primes none (the row kernel ORs their class
+// masks), and primes with p > J and p >= 10^DJ, which mark at most one j
+// per condition and row, one item for both conditions (no mark loop).
+// Item = index | kind << 16 | part << 18.
 template <int DJ>
 __global__ __launch_bounds__(1024)
 void td_items_kernel(const uint32_t *__restrict__ tbl, uint32_t *__restrict__ ctrl, uint32_t J,
@@ -321,19 +323,22 @@ void td_items_kernel(const uint32_t *__restrict__ tbl, uint32_t *__restrict__ ct
     const uint32_t per = (count + 1023u) / 1024u;
     const uint32_t i0 = min(threadIdx.x * per, count), i1 = min(i0 + per, count);
     const uint32_t lim = td_pow10(DJ);
-    auto parts = [&](uint32_t i, uint32_t &nn, uint32_t &nr) {
+    // one item of kind TI_WHOLE or TI_ONE (returned), or nn + nr parts (TI_N)
+    auto parts = [&](uint32_t i, uint32_t &nn, uint32_t &nr) -> uint32_t {
         const uint32_t p = tbl[TT_P * TD_SMALL_CAP + i];
-        if (p == 2u || p == 5u) { nn = 0; nr = 0; return true; }
-        if (td_fill_index(p) >= 0) { nn = 0; nr = 0; return false; }   // word fills
+        nn = nr = 0;
+        if (p == 2u || p == 5u) return TI_WHOLE;
+        if (td_fill_index(p) >= 0) return TI_N;                          // word fills
+        if (p > J && p >= lim) return TI_ONE;
         const uint32_t flags = tbl[TT_FLAGS * TD_SMALL_CAP + i];
         nn = (flags & 1u) ? ((J + p - 1u) / p + TD_CH - 1u) / TD_CH : 0u;
         nr = ((lim - 1u + p - 1u) / p + TD_CH - 1u) / TD_CH;
-        return false;
+        return TI_N;
     };
     uint32_t mine = 0;
     for (uint32_t i = i0; i < i1; i++) {
         uint32_t nn, nr;
-        mine += parts(i, nn, nr) ? 1u : nn + nr;
+        mine += parts(i, nn, nr) != TI_N ? 1u : nn + nr;
     }
     scan[threadIdx.x] = mine;
     __syncthreads();
@@ -352,8 +357,9 @@ void td_items_kernel(const uint32_t *__restrict__ tbl, uint32_t *__restrict__ ct
     uint32_t at = scan[threadIdx.x] - mine;
     for (uint32_t i = i0; i < i1; i++) {
         uint32_t nn, nr;
-        if (parts(i, nn, nr)) {
-            items[at++] = i | (uint32_t)TI_WHOLE << 16;
+        const uint32_t kind = parts(i, nn, nr);
+        if (kind != TI_N) {
+            items[at++] = i | kind << 16;
             continue;
         }
         for (uint32_t q = 0; q < nn; q++) items[at++] = i | (uint32_t)TI_N << 16 | q << 18;
@@ -494,6 +500,24 @@ void td_rows_bal_kernel(const uint32_t *__restrict__ tbl, const uint32_t *__rest
                     continue;
                 }
                 const uint32_t p = e[TT_P * TD_SMALL_CAP], magic = e[TT_MAGIC * TD_SMALL_CAP];
+                if (kind == TI_ONE) {       // p > J, p >= 10^DJ: at most one mark each
+                    const uint32_t flags = e[TT_FLAGS * TD_SMALL_CAP];
+                    if (flags & 1u) {
+                        const uint32_t u = pf_submod(e[TT_UN0 * TD_SMALL_CAP],
+                                                     td_mod(k * e[TT_B * TD_SMALL_CAP], p, magic), p);
+                        if (u < J) td_lds_mark(lds, rowbit + u + 1u);
+                    }
+                    if ((flags >> fbit) & 1u) {
+                        const uint32_t ev = e[(TT_E + (lk - 1) * 4 + (DJ - 1)) * TD_SMALL_CAP];
+                        const uint32_t u = pf_submod(e[(TT_CM + DJ - 1) * TD_SMALL_CAP],
+                                                     td_mod(rk * ev, p, magic), p);
+                        if (u + 1u < lim) {
+                            const uint32_t j = rdec[u + 1u];
+                            if (j <= J) td_lds_mark(lds, rowbit + j);
+                        }
+                    }
+                    continue;
+                }
                 if (kind == TI_N) {
                     const uint32_t u = pf_submod(e[TT_UN0 * TD_SMALL_CAP],
                                                  td_mod(k * e[TT_B * TD_SMALL_CAP], p, magic), p);
