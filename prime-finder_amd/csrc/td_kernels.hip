@@ -198,34 +198,75 @@ void td_small_setup_kernel(const uint32_t *__restrict__ small,
          i += gridDim.x * TD_BLOCK) {
         const uint32_t p = small[i];
         uint32_t *e = tbl + i;          // field f at e[f * TD_SMALL_CAP]
-        const uint32_t ra = td_powmod16(10u, (uint64_t)a, p);
-        const uint32_t rb = td_powmod16(10u, (uint64_t)b, p);
-        const uint32_t rc = td_powmod16(10u, (uint64_t)c, p);
-        const uint32_t ra1 = (ra + p - 1u) % p;
+        const uint32_t magic = (uint32_t)(0x100000000ull / p);
+        // x y mod p for x, y < p < 2^16 (the product fits 32 bits)
+        auto mulm = [&](uint32_t x, uint32_t y) { return td_mod(x * y, p, magic); };
+        // The eleven powers of ten a prime needs (10^a, 10^b, 10^c and
+        // 10^(a-b-l), 10^(a-c-l) for l = 1..4), sharing their squarings:
+        // right to left over the exponents' bits, which are the same in
+        // every lane.  These are the residues of the reference's powmod
+        // (trial_div_cpu.c:22-30), 10^0 = 1 and p = 2, 5 included; negative
+        // exponents are gated below.
+        uint64_t ex[11] = {(uint64_t)a, (uint64_t)b, (uint64_t)c};
+        for (int l = 1; l <= 4; l++) {
+            ex[2 + l] = a - b - l >= 0 ? (uint64_t)(a - b - l) : 0u;
+            ex[6 + l] = a - c - l >= 0 ? (uint64_t)(a - c - l) : 0u;
+        }
+        uint64_t top = 0;
+        uint32_t pw[11];
+#pragma unroll
+        for (int q = 0; q < 11; q++) {
+            pw[q] = 1u % p;
+            top |= ex[q];
+        }
+        uint32_t sq = 10u % p;
+        for (int bit = 0; (top >> bit) != 0u; bit++) {
+#pragma unroll
+            for (int q = 0; q < 11; q++)
+                if ((ex[q] >> bit) & 1u) pw[q] = mulm(pw[q], sq);
+            if ((top >> (bit + 1)) != 0u) sq = mulm(sq, sq);
+        }
+        const uint32_t ra = pw[0], rb = pw[1], rc = pw[2];
+        uint32_t kres[4], jr[4];
+#pragma unroll
+        for (int l = 1; l <= 4; l++) {
+            kres[l - 1] = a - b - l >= 0 ? pw[2 + l] : 0u;
+            jr[l - 1] = a - c - l >= 0 ? pw[6 + l] : 0u;
+        }
+        // Fermat inverses of rc and the jr (exponent p - 2, per lane), five
+        // independent chains in one loop; a zero base has no inverse and is
+        // gated (the reference only inverts nonzero powers)
+        uint32_t ib[5] = {rc, jr[0], jr[1], jr[2], jr[3]}, iv[5] = {1u, 1u, 1u, 1u, 1u};
+        for (uint32_t ee = p - 2u; ee != 0u; ee >>= 1) {
+#pragma unroll
+            for (int q = 0; q < 5; q++) {
+                if (ee & 1u) iv[q] = mulm(iv[q], ib[q]);
+                ib[q] = mulm(ib[q], ib[q]);
+            }
+        }
+        const uint32_t ra1 = ra ? ra - 1u : p - 1u;
         uint32_t flags = 0;
         uint32_t un0 = 0, bstep = 0;
         if (rc != 0u) {
-            const uint32_t ic = td_powmod16(rc, p - 2u, p);
-            un0 = (ra1 * ic % p + p - 1u) % p;   // ((ra-1)/rc - 1) mod p
-            bstep = rb * ic % p;                 // rb/rc mod p
+            const uint32_t x = mulm(ra1, iv[0]);
+            un0 = x ? x - 1u : p - 1u;            // ((ra-1)/rc - 1) mod p
+            bstep = mulm(rb, iv[0]);              // rb/rc mod p
             flags |= 1u;
         }
-        uint32_t kres[4], jinv[4];
-        for (int l = 1; l <= 4; l++) {
-            const long long eb = a - b - l, ec = a - c - l;
-            kres[l - 1] = eb >= 0 ? td_powmod16(10u, (uint64_t)eb, p) : 0u;
-            const uint32_t jr = ec >= 0 ? td_powmod16(10u, (uint64_t)ec, p) : 0u;
-            jinv[l - 1] = jr ? td_powmod16(jr, p - 2u, p) : 0u;
-        }
+        uint32_t jinv[4];
+#pragma unroll
+        for (int l = 0; l < 4; l++) jinv[l] = jr[l] ? iv[1 + l] : 0u;
         e[TT_P * TD_SMALL_CAP] = p;
-        e[TT_MAGIC * TD_SMALL_CAP] = (uint32_t)(0x100000000ull / p);
+        e[TT_MAGIC * TD_SMALL_CAP] = magic;
         e[TT_UN0 * TD_SMALL_CAP] = un0;
         e[TT_B * TD_SMALL_CAP] = bstep;
-        for (int lj = 0; lj < 4; lj++)
-            e[(TT_CM + lj) * TD_SMALL_CAP] = (ra1 * jinv[lj] % p + p - 1u) % p;
+        for (int lj = 0; lj < 4; lj++) {
+            const uint32_t x = mulm(ra1, jinv[lj]);
+            e[(TT_CM + lj) * TD_SMALL_CAP] = x ? x - 1u : p - 1u;
+        }
         for (int lk = 0; lk < 4; lk++)
             for (int lj = 0; lj < 4; lj++) {
-                e[(TT_E + lk * 4 + lj) * TD_SMALL_CAP] = kres[lk] * jinv[lj] % p;
+                e[(TT_E + lk * 4 + lj) * TD_SMALL_CAP] = mulm(kres[lk], jinv[lj]);
                 if (kres[lk] != 0u && jinv[lj] != 0u) flags |= 1u << (1 + lk * 4 + lj);
             }
         e[TT_FLAGS * TD_SMALL_CAP] = flags;
