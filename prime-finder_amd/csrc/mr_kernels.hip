@@ -2553,19 +2553,22 @@ int enqueue(mr_sched *S, std::vector<Req> &add)
 extern "C" int mr_sched_submit(mr_sched_t *S, const mr_candidate_t *cands, const int64_t *tags, int n)
 {
     if (!S || (!cands && n) || n < 0) return -1;
-    std::vector<Req> add((size_t)n);
-    for (int i = 0; i < n; i++)
-        if (!make_req(cands[i], tags ? tags[i] : (int64_t)(S->reqs.size() + i), add[i])) return -1;
-    return enqueue(S, add);
+    try {
+        std::vector<Req> add((size_t)n);
+        for (int i = 0; i < n; i++)
+            if (!make_req(cands[i], tags ? tags[i] : (int64_t)(S->reqs.size() + i), add[i]))
+                return -1;
+        return enqueue(S, add);
+    } catch (...) {             // allocation failure: nothing escapes the C ABI
+        return -2;
+    }
 }
 
-extern "C" int mr_sched_submit_survivors(mr_sched_t *S, long long a, long long b, long long c,
-                                         const uint32_t *d_list, uint32_t n, int which,
-                                         int64_t tag_base)
+namespace {
+
+int submit_survivors(mr_sched *S, long long a, long long b, long long c, const uint32_t *d_list,
+                     uint32_t n, int which, int64_t tag_base)
 {
-    if (!S || (!d_list && n) || which < 0 || which > 2 || a < 1 || b < 0 || c < 0) return -1;
-    if (n == 0) return 0;
-    if (hipSetDevice(S->device) != hipSuccess) return -2;
     std::vector<Req> add((size_t)n);
     if (std::max(b, c) > a - 7) {
         // k 10^b + j 10^c may reach the leading digit, so the decimal length
@@ -2594,6 +2597,23 @@ extern "C" int mr_sched_submit_survivors(mr_sched_t *S, long long a, long long b
     }
     return enqueue(S, add);
 }
+
+} // namespace
+
+extern "C" int mr_sched_submit_survivors(mr_sched_t *S, long long a, long long b, long long c,
+                                         const uint32_t *d_list, uint32_t n, int which,
+                                         int64_t tag_base)
+{
+    if (!S || (!d_list && n) || which < 0 || which > 2 || a < 1 || b < 0 || c < 0) return -1;
+    if (n == 0) return 0;
+    if (hipSetDevice(S->device) != hipSuccess) return -2;
+    try {
+        return submit_survivors(S, a, b, c, d_list, n, which, tag_base);
+    } catch (...) {             // allocation failure: nothing escapes the C ABI
+        return -2;
+    }
+}
+
 
 namespace {
 
@@ -2750,9 +2770,25 @@ extern "C" int mr_sched_restore(mr_sched_t *S, const char *path)
     }
 }
 
+namespace {
+int sched_pump(mr_sched *S, int block);
+}
+
 extern "C" int mr_sched_pump(mr_sched_t *S, int block)
 {
     if (!S) return -1;
+    try {
+        return sched_pump(S, block);
+    } catch (...) {             // allocation failure: nothing escapes the C ABI
+        S->failed = true;
+        return -2;
+    }
+}
+
+namespace {
+
+int sched_pump(mr_sched *S, int block)
+{
     if (S->failed) return -2;
     if (hipSetDevice(S->device) != hipSuccess) return -2;
     auto fail = [&](int rc) { S->failed = true; return rc < 0 ? rc : -2; };
@@ -2813,6 +2849,8 @@ extern "C" int mr_sched_pump(mr_sched_t *S, int block)
         block = 0;      // one wait per call; loop once more to act on it
     }
 }
+
+} // namespace
 
 extern "C" int mr_sched_results(mr_sched_t *S, mr_result_t *out, int cap)
 {
@@ -2955,23 +2993,28 @@ extern "C" int mr_test_batch(const mr_candidate_t *cands, int n, const uint32_t 
     if (!cands || n < 1 || !verdict) return -1;
     mr_sched_t *S = mr_sched_create(bases, n_bases, device, 0, 0, 0);
     if (!S) return -1;
-    std::vector<int64_t> tags((size_t)n);
-    for (int i = 0; i < n; i++) tags[i] = i;
-    int rc = mr_sched_submit(S, cands, tags.data(), n);
-    while (rc == 0) {
-        const int left = mr_sched_pump(S, 1);
-        if (left < 0) { rc = left; break; }
-        if (left == 0) break;
-    }
-    if (rc == 0) {
-        std::vector<mr_result_t> res((size_t)n);
-        const int k = mr_sched_results(S, res.data(), n);
-        for (int i = 0; i < k; i++) {
-            const int64_t t = res[i].tag;
-            verdict[t] = (uint8_t)res[i].verdict;
-            if (rounds) rounds[t] = res[i].rounds;
+    int rc = 0;
+    try {
+        std::vector<int64_t> tags((size_t)n);
+        for (int i = 0; i < n; i++) tags[i] = i;
+        rc = mr_sched_submit(S, cands, tags.data(), n);
+        while (rc == 0) {
+            const int left = mr_sched_pump(S, 1);
+            if (left < 0) { rc = left; break; }
+            if (left == 0) break;
         }
-        if (k != n) rc = -2;
+        if (rc == 0) {
+            std::vector<mr_result_t> res((size_t)n);
+            const int k = mr_sched_results(S, res.data(), n);
+            for (int i = 0; i < k; i++) {
+                const int64_t t = res[i].tag;
+                verdict[t] = (uint8_t)res[i].verdict;
+                if (rounds) rounds[t] = res[i].rounds;
+            }
+            if (k != n) rc = -2;
+        }
+    } catch (...) {             // allocation failure: nothing escapes the C ABI
+        rc = -2;
     }
     mr_sched_destroy(S);
     return rc;
