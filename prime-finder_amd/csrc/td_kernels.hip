@@ -359,7 +359,7 @@ __global__ __launch_bounds__(1024)
 void td_items_kernel(const uint32_t *__restrict__ tbl, uint32_t *__restrict__ ctrl, uint32_t J,
                      uint32_t *__restrict__ items)
 {
-    __shared__ uint32_t scan[1024];
+    __shared__ uint32_t scan[16];
     const uint32_t count = min(ctrl[CT_SMALL_COUNT], TD_SMALL_CAP);
     const uint32_t per = (count + 1023u) / 1024u;
     const uint32_t i0 = min(threadIdx.x * per, count), i1 = min(i0 + per, count);
@@ -381,21 +381,27 @@ void td_items_kernel(const uint32_t *__restrict__ tbl, uint32_t *__restrict__ ct
         uint32_t nn, nr;
         mine += parts(i, nn, nr) != TI_N ? 1u : nn + nr;
     }
-    scan[threadIdx.x] = mine;
-    __syncthreads();
-    for (uint32_t d = 1; d < 1024u; d <<= 1) {          // inclusive scan
-        const uint32_t v = threadIdx.x >= d ? scan[threadIdx.x - d] : 0u;
-        __syncthreads();
-        scan[threadIdx.x] += v;
-        __syncthreads();
+    // exclusive offsets: a scan within each wave by lane shuffles, then the
+    // 16 wave totals through LDS (one barrier)
+    const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+    uint32_t incl = mine;
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        const uint32_t t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
     }
-    const uint32_t total = scan[1023];
+    if (lane == 63u) scan[wid] = incl;
+    __syncthreads();
+    uint32_t total = 0;
+    for (uint32_t w = 0; w < 16u; w++) {
+        if (w == wid) incl += total;
+        total += scan[w];
+    }
     if (threadIdx.x == 0) {
         ctrl[CT_ITEMS] = total;
         ctrl[CT_ITEM_OVF] = total > TD_ITEM_CAP ? 1u : 0u;
     }
     if (total > TD_ITEM_CAP) return;
-    uint32_t at = scan[threadIdx.x] - mine;
+    uint32_t at = incl - mine;
     for (uint32_t i = i0; i < i1; i++) {
         uint32_t nn, nr;
         const uint32_t kind = parts(i, nn, nr);
@@ -1056,6 +1062,9 @@ struct TdScratch {
     uint32_t *items = nullptr;   // [TD_ITEM_CAP] row-kernel work items
     uint32_t *mid = nullptr;     // [TD_MID_CAP] compacted mid-range primes
     uint32_t *fill = nullptr;    // [TD_FILL_WORDS] class masks of the fill primes
+    uint32_t fill_J = 0;         // the J the class masks were built for (0: none)
+    uint32_t *fill_key = nullptr; // where that J is kept across calls (this struct's
+                                  // own field, or the allocation-free state's)
     hipStream_t aux = nullptr;   // row kernel, concurrent with the lane kernel
     hipEvent_t fork = nullptr, join = nullptr;
 };
@@ -1106,6 +1115,7 @@ int td_scratch(TdScratch **out)
             return -1;
         }
         s.ready = true;
+        s.fill_key = &s.fill_J;
     }
     *out = &s;
     return 0;
@@ -1133,8 +1143,13 @@ void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long
     const hipStream_t rs = s.aux ? s.aux : st;
     if (a - c >= DJ) {
         td_items_kernel<DJ><<<1, 1024, 0, st>>>(s.tbl, s.ctrl, J, s.items);
-        const uint32_t fw = 2u * TD_FILL_CLASSES * ((J + 31u) >> 5);
-        td_fill_table_kernel<DJ><<<(fw + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(J, s.fill);
+        // the class masks depend on J alone: built once per device and J
+        // (calls on one device are ordered, as they share the scratch)
+        if (!s.fill_key || *s.fill_key != J) {
+            const uint32_t fw = 2u * TD_FILL_CLASSES * ((J + 31u) >> 5);
+            td_fill_table_kernel<DJ><<<(fw + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(J, s.fill);
+            if (s.fill_key) *s.fill_key = J;
+        }
     }
     if (s.aux) {
         (void)hipEventRecord(s.fork, st);
@@ -1194,6 +1209,7 @@ static __device__ __align__(256) uint32_t scratch[TD_NA_WORDS];
 
 struct TdNoAlloc {
     uint32_t *ptr = nullptr;     // device address of `scratch` on this device
+    uint32_t fill_J = 0;         // the J its class masks were built for (0: none)
 };
 TdNoAlloc g_noalloc[TD_MAX_DEV];
 std::mutex g_noalloc_mu;
@@ -1227,6 +1243,7 @@ int td_noalloc_scratch(hipStream_t st, TdScratch *out)
     out->items = na.ptr + TD_NA_ITEMS;
     out->mid = na.ptr + TD_NA_MID;
     out->fill = na.ptr + TD_NA_FILL;
+    out->fill_key = &na.fill_J;
     return 0;
 }
 
@@ -1264,7 +1281,11 @@ int td_launch_with(const uint32_t *d_primes, uint32_t n, long long a, long long 
     case 3: td_launch_rest<3>(d_primes, n, a, b, c, K, J, d_bits, *s, st, ev0, ev1); break;
     default: td_launch_rest<4>(d_primes, n, a, b, c, K, J, d_bits, *s, st, ev0, ev1); break;
     }
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+    if (hipGetLastError() != hipSuccess) {
+        if (s->fill_key) *s->fill_key = 0u;      // the class masks may not have been built
+        return -1;
+    }
+    return 0;
 }
 
 // Device copy of a host prime list, reused while the caller's list is
