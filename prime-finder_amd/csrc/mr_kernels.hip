@@ -2558,6 +2558,11 @@ extern "C" int mr_sched_submit(mr_sched_t *S, const mr_candidate_t *cands, const
         for (int i = 0; i < n; i++)
             if (!make_req(cands[i], tags ? tags[i] : (int64_t)(S->reqs.size() + i), add[i]))
                 return -1;
+        // longest first within one call (the classical rule for finishing a
+        // batch early on identical units: the longest tests start while the
+        // short ones fill the gaps at the end); calls keep their order
+        std::stable_sort(add.begin(), add.end(),
+                         [](const Req &x, const Req &y) { return x.L > y.L; });
         return enqueue(S, add);
     } catch (...) {             // allocation failure: nothing escapes the C ABI
         return -2;
@@ -2813,15 +2818,18 @@ int sched_pump(mr_sched *S, int block)
         Pool *order[MAX_LOGM + 1];
         int np = 0;
         for (Pool *p : S->pools) if (p) order[np++] = p;
-        for (int a = 1; a < np; a++)       // by submit time of the head request
-            for (int b = a; b > 0; b--) {
+        for (int a = 1; a < np; a++)       // by submit time of the head request,
+            for (int b = a; b > 0; b--) {  // the longer transform first on a tie
                 auto head = [&](Pool *p) {
                     const int lg = p->logm;
                     return S->qhead[lg] < S->queue[lg].size()
                                ? S->reqs[S->queue[lg][S->qhead[lg]]].t_submit : 1e300;
                 };
-                if (head(order[b]) < head(order[b - 1])) std::swap(order[b], order[b - 1]);
-                else break;
+                const double hb = head(order[b]), ha = head(order[b - 1]);
+                if (hb < ha || (hb == ha && order[b]->logm > order[b - 1]->logm))
+                    std::swap(order[b], order[b - 1]);
+                else
+                    break;
             }
         for (int i = 0; i < np; i++) {
             Pool *p = order[i];
