@@ -7,7 +7,9 @@ rounding of x, of A, of the reciprocal (v_rcp_f32 is accurate to 1 ulp,
 the host's division to half an ulp) and of the product stays below
 5 * 2^-24 relative, so the biased estimate never exceeds floor(x / A) and,
 while it is below 2^18, is at most one less; one conditional subtraction
-then gives x mod A exactly.  Larger estimates take the exact division.
+then gives x mod A exactly, and one conditional increment the quotient
+(the return-time computation's runs, td_div_est).  Larger estimates take
+the exact division.
 
 Replayed here in numpy binary32 arithmetic with the reciprocal rounded to
 nearest and perturbed by one ulp either way, on random and edge operands
@@ -62,6 +64,20 @@ def test_biased_estimate_bounds_and_exact_remainder():
         d = (r - A) & 0xFFFFFFFF
         rem = np.minimum(r, d)
         assert (rem[fast] == (x % A)[fast]).all()
+
+
+def test_quotient_form_of_the_return_time_steps():
+    """td_div_est (the return-time computation's runs): the estimate plus
+    one when x - est Y >= Y is floor(x / Y) whenever est < 2^18."""
+    rng = np.random.default_rng(99)
+    x, A = operands(rng, 400000)
+    q = x // A
+    for shift in (-1, 0, 1):
+        est = estimate(x, A, shift)
+        fast = est < (1 << 18)
+        r = (x - est * A) & 0xFFFFFFFF
+        got = est + (r >= A)
+        assert (got[fast] == q[fast]).all()
 
 
 def test_unbiased_estimate_breaks_the_bound():
