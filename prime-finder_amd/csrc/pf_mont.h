@@ -95,7 +95,8 @@ PF_HD uint32_t pf_mont_pow(uint32_t base, uint64_t e, const PfMont &m)
 
 // base^e[k] (Montgomery form) for NE exponents that are the same in every
 // lane, sharing the squarings of base: right to left over the bits of the
-// largest exponent, one squaring per bit and one product per set bit.
+// largest exponent, one squaring per bit and one product per set bit after
+// an exponent's lowest (which takes the power itself: R * sq / R = sq).
 template <int NE>
 PF_HD void pf_mont_pow_multi(uint32_t base, const uint64_t (&e)[NE], uint32_t (&out)[NE],
                              const PfMont &m)
@@ -108,7 +109,8 @@ PF_HD void pf_mont_pow_multi(uint32_t base, const uint64_t (&e)[NE], uint32_t (&
     uint32_t sq = base;
     for (int i = 0; (top >> i) != 0; i++) {
         for (int k = 0; k < NE; k++)
-            if ((e[k] >> i) & 1u) out[k] = pf_mont_mul(out[k], sq, m);
+            if ((e[k] >> i) & 1u)
+                out[k] = (e[k] & ((1ull << i) - 1u)) == 0u ? sq : pf_mont_mul(out[k], sq, m);
         if ((top >> (i + 1)) != 0) sq = pf_mont_mul(sq, sq, m);
     }
 }

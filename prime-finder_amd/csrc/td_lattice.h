@@ -99,10 +99,18 @@ PF_HD uint32_t td_rdec_arith(uint32_t R, int d)
 // arithmetic: the result is in [1, p)), and x = w sig^-1 mod p.  A composite
 // list entry (gcd(sig, p) > 1) can make A divide M or vanish; the search
 // then ends with "no hit" instead of looping.
+//
+// A query that only needs a hit below W stops early: every solution of an
+// undecided level wraps (its next-level unknown is at least 1), and the
+// unknowns of successive levels satisfy x_i >= x_(i+1) M_i / A_i with
+// M_(i+1) = A_i, so after undecided levels 0 .. k the product telescopes to
+// x >= p / A_k.  Once A_k W <= p no hit lies in [0, W): the query ends with
+// "no hit" there, without descending to its exact first hit.
 struct TdFh {
     uint32_t c;                 // the query's start value
     uint32_t L, A, M, tacc, w;
-    bool neg, live, fail;       // live: levels still to run
+    bool neg, live, fail;       // live: levels still to run; fail: no hit (none
+                                // below the window, or a composite entry)
     bool imm;                   // c < T: t = 0 is the hit
 };
 
@@ -152,6 +160,14 @@ PF_HD uint32_t td_mod_est(uint32_t x, uint32_t A, float rA, uint32_t &est)
     return d < r ? d : r;
 }
 
+// floor(p / W) or less, for p < 2^31 and W >= 1: a query's early-exit
+// threshold on A (the biased reciprocal's estimate never exceeds the
+// quotient, and a lower threshold only exits later).
+PF_HD uint32_t td_window_thr(uint32_t p, uint32_t W)
+{
+    return (uint32_t)((float)p * td_rcp_lo(W));
+}
+
 // The levels of a started query, until it is decided (or failed after
 // TD_FH_MAX_LEVELS: the bound is the same in every lane, so the loop
 // counter is wave-uniform).  Per level, with A > M/2 first mirrored:
@@ -159,8 +175,9 @@ PF_HD uint32_t td_mod_est(uint32_t x, uint32_t A, float rA, uint32_t &est)
 // a partial quotient of 2^18 or more, which is rare), then either the next
 // multiple of A lands in [L, L + T - 1] (decided) or the level moves to
 // (A - M mod A, A, L mod A).  (M mod A == 0 makes the next level's A
-// vanish, which fails there.)
-PF_HD void td_fh_run(TdFh &q, uint32_t T)
+// vanish, which fails there.)  athr = td_window_thr(p, W) ends the query
+// with "no hit" once the first hit is known to be W or more (0: never).
+PF_HD void td_fh_run(TdFh &q, uint32_t T, uint32_t athr)
 {
     if (!q.live) return;
     uint32_t L0 = q.L, A0 = q.A, M = q.M, tacc = q.tacc, w = 0u;
@@ -202,6 +219,12 @@ PF_HD void td_fh_run(TdFh &q, uint32_t T)
         M = A;
         A0 = A - mA;
         L0 = rL;
+        if (M <= athr) {                         // first hit >= p / A >= W
+            TD_LAT_COUNT(query);
+            live = false;
+            fail = true;
+            break;
+        }
     }
     if (live) TD_LAT_COUNT(query);
     q.tacc = tacc;
@@ -227,15 +250,16 @@ PF_HD uint32_t td_fh_result(const TdFh &q, uint32_t siginvM, const PfMont &m, ui
     return pf_mont_mul(wt, siginvM, m);
 }
 
-// Smallest t >= 0 with (c + t sig) mod p < T, and y = that residue;
-// 0xFFFFFFFF when there is none (only for a composite list entry).
-// siginvM = sig^-1 in Montgomery form.
-PF_HD uint32_t td_first_hit(uint32_t c, uint32_t sig, uint32_t T, uint32_t siginvM,
-                            const PfMont &m, uint32_t &y)
+// Smallest t >= 0 with (c + t sig) mod p < T, and y = that residue, when
+// that t is below W (W >= 1); otherwise t >= W or 0xFFFFFFFF (no hit at
+// all: only for a composite list entry).  siginvM = sig^-1 in Montgomery
+// form.
+PF_HD uint32_t td_first_hit(uint32_t c, uint32_t sig, uint32_t T, uint32_t W,
+                            uint32_t siginvM, const PfMont &m, uint32_t &y)
 {
     TdFh q;
     td_fh_start(q, c, sig, T, m.p);
-    td_fh_run(q, T);
+    td_fh_run(q, T, td_window_thr(m.p, W));
     return td_fh_result(q, siginvM, m, y);
 }
 
@@ -401,23 +425,23 @@ PF_HD void td_lattice_setup(uint32_t p, long long a, long long b, long long c, u
     const uint32_t tenInv = (uint32_t)(X >> 1) * 0xCCCCCCCDu;
 
     // Montgomery-form powers (suffix M) and plain residues; the powers of
-    // 10 and of 10^-1 share their squarings.
+    // 10 and of 10^-1 share their squarings, and 10^(b-c) and 10^(c-b) are
+    // one exponent |b - c| in each chain.
     const long long eb1 = a - b - 1;
-    uint32_t pw[4], nw[3];
+    const uint64_t bc = (uint64_t)(c <= b ? b - c : c - b);
+    uint32_t pw[2], nw[3];
     {
-        const uint64_t pe[4] = {(uint64_t)a, (uint64_t)b, (uint64_t)(eb1 >= 0 ? eb1 : 0),
-                                (uint64_t)(c >= b ? c - b : 0)};
-        const uint64_t ne[3] = {(uint64_t)c, (uint64_t)(a - c - DJ), (uint64_t)(c < b ? b - c : 0)};
-        pf_mont_pow_multi<4>(ten, pe, pw, m);
+        const uint64_t pe[2] = {(uint64_t)a, bc};
+        const uint64_t ne[3] = {(uint64_t)c, (uint64_t)(a - c - DJ), bc};
+        pf_mont_pow_multi<2>(ten, pe, pw, m);
         pf_mont_pow_multi<3>(tenInv, ne, nw, m);
     }
     const uint32_t raM = pw[0];                                                  // 10^a
-    const uint32_t rbM = pw[1];                                                  // 10^b
     const uint32_t icM = nw[0];                                                  // 10^-c
     const uint32_t ra1M = pf_submod(raM, m.one, p);                              // 10^a - 1
-    const uint32_t bstepM = pf_mont_mul(rbM, icM, m);                            // 10^(b-c)
+    const uint32_t bstepM = c <= b ? pw[1] : nw[2];                              // 10^(b-c)
+    const uint32_t binvM = c <= b ? nw[2] : pw[1];                               // 10^(c-b)
     const uint32_t bstep = pf_mont_out(bstepM, m);
-    const uint32_t binvM = c >= b ? pw[3] : nw[2];                               // 10^(c-b)
 
     // N: (k, j) with j - 1 = (un1 - (k - 1) bstep) mod p < J
     r.p = p;
@@ -429,25 +453,35 @@ PF_HD void td_lattice_setup(uint32_t p, long long a, long long b, long long c, u
     r.cm = 0u;
     for (int i = 0; i < TD_LAT_CONDS - 1; i++) r.sig[i] = r.sinv[i] = 0u;
 
-    // rev: per digit count lk of k, R - 1 = (cm - rev(k) E_lk) mod p
+    // rev: per digit count lk of k, R - 1 = (cm - rev(k) E_lk) mod p with
+    // E_lk = 10^(a-b-lk) / 10^(a-c-DJ) = 10^(c-b+DJ-lk), for the classes
+    // lk = 1 .. LK that exist on the grid and keep a rev term (a - b >= lk)
     if (eb1 < 0) return;                                    // no class has a rev term
     const uint32_t jinvM = nw[1];                           // 10^-(a-c-DJ)
     const uint32_t cm = pf_submod(pf_mont_out(pf_mont_mul(ra1M, jinvM, m), m), 1u, p);
     r.cm = cm;
-    uint32_t kresM = pw[2];                                 // 10^(a-b-lk)
     const uint32_t T = td_pow10(DJ) - 1u;
     const int DK = (K >= 1000u) ? 4 : (K >= 100u) ? 3 : (K >= 10u) ? 2 : 1;
-    // E_lk = 10^(a-b-lk) / 10^(a-c-DJ); E_lk^-1 = 10^(b-c) 10^(lk-DJ)
-    uint32_t einvM = bstepM;
-    for (int d = 1; d < DJ; d++) einvM = pf_mont_mul(einvM, tenInv, m);
+    const int LK = a - b < DK ? (int)(a - b) : DK;
+    // E_LK and E_LK^-1 from 10^(c-b) and 10^(b-c), then one factor of 10
+    // per class downwards
+    uint32_t eM = binvM, einvM = bstepM;
+    for (int d = LK; d < DJ; d++) {
+        eM = pf_mont_mul(eM, ten, m);
+        einvM = pf_mont_mul(einvM, tenInv, m);
+    }
+    for (int d = DJ; d < LK; d++) {
+        eM = pf_mont_mul(eM, tenInv, m);
+        einvM = pf_mont_mul(einvM, ten, m);
+    }
     // (unrolled: the record's arrays are indexed by constants, so they stay
     // in registers)
 #if defined(__HIPCC__)
 #pragma unroll
 #endif
-    for (int lk = 1; lk <= TD_LAT_CONDS; lk++) {
-        if (lk > DK || a - b - lk < 0) break;
-        const uint32_t E = pf_mont_out(pf_mont_mul(kresM, jinvM, m), m);
+    for (int lk = TD_LAT_CONDS; lk >= 1; lk--) {
+        if (lk > LK) continue;
+        const uint32_t E = pf_mont_out(eM, m);
         if (lk == 1) {
             uint32_t v = cm;
             for (uint32_t rr = 1u; rr < 10u; rr++) {
@@ -462,9 +496,9 @@ PF_HD void td_lattice_setup(uint32_t p, long long a, long long b, long long c, u
             r.sig[lk - 2] = p - E;
             r.sinv[lk - 2] = p - einvM;
             r.flags |= 1u << (lk - 1);
+            eM = pf_mont_mul(eM, ten, m);
+            einvM = pf_mont_mul(einvM, tenInv, m);
         }
-        kresM = pf_mont_mul(kresM, tenInv, m);
-        einvM = pf_mont_mul(einvM, ten, m);
     }
 }
 
@@ -487,7 +521,7 @@ PF_HD void td_lattice_run(const TdLatRec &r, int cond, const TdLatItem &it, uint
     const bool many = 2ull * it.W * it.T >= r.p;
     uint32_t cc = it.c0, t0 = 0u, t = 0u, y = 0u;
     for (int n = 0;; n++) {
-        const uint32_t x = td_first_hit(cc, it.sig, it.T, it.sinv, m, y);
+        const uint32_t x = td_first_hit(cc, it.sig, it.T, it.W - t0, it.sinv, m, y);
         if (x >= it.W - t0) return;                         // (no overflow: t0 < W)
         t = t0 + x;
         td_lattice_mark<DJ>(cond, t, y, K, J, mark);

@@ -64,3 +64,40 @@ extern "C" int gaps_rule_mismatches(uint32_t p, uint32_t sig, uint32_t T)
     }
     return bad;
 }
+
+// The windowed first-hit query td_first_hit against brute force: for n
+// start values c (c0, c0 + 1, ...), the first t >= 0 with (c + t sig) mod p
+// < T; when that t is below W the query must return it and its residue,
+// otherwise any t >= W (the early exit) or none.  p an odd prime, 1 <= sig
+// < p, 1 <= T < p, W >= 1.  Returns the number of mismatches.
+extern "C" int first_hit_mismatches(uint32_t p, uint32_t sig, uint32_t T, uint32_t W,
+                                    uint32_t c0, uint32_t n)
+{
+    const PfMont m = pf_mont_init(p);
+    // sig^-1 in Montgomery form: sig^(p-2) R mod p
+    const uint32_t sinvM = pf_mont_pow(pf_mont_in(sig, m), (uint64_t)p - 2u, m);
+    int bad = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t c = (uint32_t)(((uint64_t)c0 + i) % p);
+        uint64_t t = 0, w = c;
+        while (w >= T) {
+            t++;
+            w = (w + sig) % p;
+        }
+        uint32_t y = 0;
+        const uint32_t got = td_first_hit(c, sig, T, W, sinvM, m, y);
+        if (t < W) {
+            if (got != t || y != w) bad++;
+        } else if (got < W) {
+            bad++;
+        }
+        // the windows just above and at the first hit: the early exit must
+        // not fire while the hit is inside, and no t below W may come back
+        if (t < 0xFFFFFFFFu) {
+            const uint32_t in = td_first_hit(c, sig, T, (uint32_t)t + 1u, sinvM, m, y);
+            if (in != t || y != w) bad++;
+            if (t > 0 && td_first_hit(c, sig, T, (uint32_t)t, sinvM, m, y) < t) bad++;
+        }
+    }
+    return bad;
+}

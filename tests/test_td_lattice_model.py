@@ -33,6 +33,8 @@ def model(tmp_path_factory):
     L.lattice_sieve.restype = C.c_int
     L.gaps_rule_mismatches.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32]
     L.gaps_rule_mismatches.restype = C.c_int
+    L.first_hit_mismatches.argtypes = [C.c_uint32] * 6
+    L.first_hit_mismatches.restype = C.c_int
     return L
 
 
@@ -124,3 +126,20 @@ def test_return_time_rule_brute_force(model):
         assert model.gaps_rule_mismatches(p, sig, T) == 0, (p, sig, T)
         cases += 1
     assert cases == 300
+
+
+def test_windowed_first_hit_brute_force(model):
+    """The first-hit query with its window (td_first_hit: a query that only
+    needs t < W stops once the descent shows the first hit is W or more):
+    exact first hit and residue whenever it lies below W, and never a t
+    below W otherwise, against brute force on random (p, sig, T, W) with
+    small p, steps next to 1 and p - 1, windows from 1 to beyond p."""
+    rng = random.Random(17)
+    primes = [p for p in range(10007, 40000) if _isprime(p)]
+    for _ in range(300):
+        p = rng.choice(primes)
+        sig = rng.choice([1, 2, p - 1, p - 2, rng.randrange(1, p)])
+        T = rng.choice([1, 2, p - 1, rng.randrange(1, 100), rng.randrange(1, p)])
+        W = rng.choice([1, 2, 10, 100, 1000, p, rng.randrange(1, p)])
+        c0 = rng.randrange(0, p)
+        assert model.first_hit_mismatches(p, sig, T, W, c0, 40) == 0, (p, sig, T, W, c0)
