@@ -1036,14 +1036,15 @@ template <int DJ>
 __global__ __launch_bounds__(TD_BLOCK)
 void td_lattice_kernel(const uint32_t *__restrict__ primes, uint32_t n,
                        long long a, long long b, long long c,
-                       uint32_t K, uint32_t J, uint32_t *__restrict__ bits, uint32_t pmin)
+                       uint32_t K, uint32_t J, uint32_t wtop, uint32_t *__restrict__ bits,
+                       uint32_t pmin)
 {
     const uint32_t i = blockIdx.x * TD_BLOCK + threadIdx.x;
     if (i >= n) return;
     const uint32_t p = primes[i];
     if (p < pmin || p >= TD_PMAX) return;
     TdDevMark mark{bits};
-    td_lattice_prime<DJ>(p, a, b, c, K, J, mark);
+    td_lattice_prime<DJ>(p, a, b, c, K, J, wtop, mark);
 }
 
 // ---------------------------------------------------------------------------
@@ -1175,7 +1176,7 @@ void td_launch_rest(const uint32_t *d_primes, uint32_t n, long long a, long long
     if (ev0) (void)hipEventRecord(ev0, st);
     if (n && a - c >= DJ) {
         td_lattice_kernel<DJ><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
-            d_primes, n, a, b, c, K, J, d_bits, TD_LAT_MIN);
+            d_primes, n, a, b, c, K, J, td_top_window(K), d_bits, TD_LAT_MIN);
     } else if (n) {
         td_large_kernel<DJ, false><<<(n + TD_BLOCK - 1u) / TD_BLOCK, TD_BLOCK, 0, st>>>(
             d_primes, n, a, b, c, K, J, s.rdec + TD_RDEC_OFF[DJ], d_bits, TD_PMAX);

@@ -370,14 +370,40 @@ struct TdLatItem {
     uint32_t c0, sig, sinv, T, W;
 };
 
+// The search window of the top rev class (l_k = D_k, the digit count of
+// K): 1 + the largest t = rev(k) over its k <= K, chosen digit by digit
+// from k's last digit (t's leading digit) on, each the largest digit that
+// some k of [10^(D_k-1), K] with the digits chosen so far can take: below
+// the top, the smallest such k is 10^(D_k-1) + the chosen digits; at the
+// top, the digits themselves.  A full class (K = 10^D_k - 1) keeps 10^D_k;
+// K = 100 leaves t = 1 (k = 100) and a window of 2 instead of 1 000.
+// K in [1, 9999].
+PF_HD uint32_t td_top_window(uint32_t K)
+{
+    const int DK = (K >= 1000u) ? 4 : (K >= 100u) ? 3 : (K >= 10u) ? 2 : 1;
+    const uint32_t lo = td_pow10(DK - 1);
+    uint32_t s = 0u, t = 0u;                 // chosen low digits of k; t so far
+    for (int m = 0; m < DK; m++) {
+        const uint32_t pm = td_pow10(m);
+        const uint32_t room = m + 1 < DK ? K - lo - s : K - s;
+        const uint32_t d = room / pm > 9u ? 9u : room / pm;
+        s += d * pm;
+        t = t * 10u + d;
+    }
+    return t + 1u;
+}
+
+// Condition cond's search; wtop = td_top_window(K) bounds the top class.
 PF_HD bool td_lattice_item(const TdLatRec &r, int cond, uint32_t K, uint32_t J, uint32_t T_rev,
-                           TdLatItem &it)
+                           uint32_t wtop, TdLatItem &it)
 {
     if (!((r.flags >> cond) & 1u)) return false;
     if (cond == 0) {
         it = TdLatItem{r.un1, r.sigN, r.sinvN, J, K};
     } else {
-        it = TdLatItem{r.cm, r.sig[cond - 1], r.sinv[cond - 1], T_rev, td_pow10(cond + 1)};
+        const bool top = td_pow10(cond + 1) > K;            // l_k = D_k
+        it = TdLatItem{r.cm, r.sig[cond - 1], r.sinv[cond - 1], T_rev,
+                       top ? wtop : td_pow10(cond + 1)};
     }
     return true;
 }
@@ -554,13 +580,14 @@ PF_HD void td_lattice_run(const TdLatRec &r, int cond, const TdLatItem &it, uint
     }
 }
 
-// Every hit of one prime, condition by condition.  (Chaining the four
+// Every hit of one prime, condition by condition (wtop =
+// td_top_window(K)).  (Chaining the four
 // searches into one loop, so that a lane whose search is exhausted starts
 // its next condition without waiting for the rest of its wave, issued 5 %
 // fewer VALU instructions but ran 3 % longer on the 4096 x 4096 grid.)
 template <int DJ, class Mark>
 PF_HD void td_lattice_prime(uint32_t p, long long a, long long b, long long c, uint32_t K,
-                            uint32_t J, Mark &mark)
+                            uint32_t J, uint32_t wtop, Mark &mark)
 {
     TdLatRec r;
     td_lattice_setup<DJ>(p, a, b, c, K, J, mark, r);
@@ -571,7 +598,7 @@ PF_HD void td_lattice_prime(uint32_t p, long long a, long long b, long long c, u
 #endif
     for (int cond = 0; cond < TD_LAT_CONDS; cond++) {
         TdLatItem it;
-        if (td_lattice_item(r, cond, K, J, td_pow10(DJ) - 1u, it))
+        if (td_lattice_item(r, cond, K, J, td_pow10(DJ) - 1u, wtop, it))
             td_lattice_run<DJ>(r, cond, it, K, J, mark);
     }
 }

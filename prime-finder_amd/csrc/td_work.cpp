@@ -36,11 +36,12 @@ void td_work_range(const uint32_t *primes, size_t lo, size_t hi, size_t stride, 
 {
     td_work_cur = &w;
     NoMark mark;
+    const uint32_t wtop = td_top_window(K);
     for (size_t i = lo; i < hi; i += stride) {
         const uint32_t p = primes[i];
         if (p < lmin || p >= 0x80000000u) continue;
         w.prime++;
-        td_lattice_prime<DJ>(p, a, b, c, K, J, mark);
+        td_lattice_prime<DJ>(p, a, b, c, K, J, wtop, mark);
     }
     td_work_cur = nullptr;
 }

@@ -19,14 +19,15 @@ extern "C" int lattice_sieve(const uint32_t *primes, uint32_t n, long long a, lo
     const int DJ = J >= 1000u ? 4 : J >= 100u ? 3 : J >= 10u ? 2 : 1;
     if (K < 1u || J < 1u || K > 9999u || J > 9999u || a - c < DJ) return -1;
     HostMark mark{bits};
+    const uint32_t wtop = td_top_window(K);
     for (uint32_t i = 0; i < n; i++) {
         const uint32_t p = primes[i];
         if (p <= 9999u || p >= 0x80000000u || (p & 1u) == 0u) continue;
         switch (DJ) {
-        case 1: td_lattice_prime<1>(p, a, b, c, K, J, mark); break;
-        case 2: td_lattice_prime<2>(p, a, b, c, K, J, mark); break;
-        case 3: td_lattice_prime<3>(p, a, b, c, K, J, mark); break;
-        default: td_lattice_prime<4>(p, a, b, c, K, J, mark); break;
+        case 1: td_lattice_prime<1>(p, a, b, c, K, J, wtop, mark); break;
+        case 2: td_lattice_prime<2>(p, a, b, c, K, J, wtop, mark); break;
+        case 3: td_lattice_prime<3>(p, a, b, c, K, J, wtop, mark); break;
+        default: td_lattice_prime<4>(p, a, b, c, K, J, wtop, mark); break;
         }
     }
     return 0;
@@ -100,4 +101,21 @@ extern "C" int first_hit_mismatches(uint32_t p, uint32_t sig, uint32_t T, uint32
         }
     }
     return bad;
+}
+
+// td_top_window(K) against 1 + the largest reversal (D_k digits) over the
+// top class's k <= K; returns the first K in [1, 9999] where they differ
+// (0: none).
+extern "C" uint32_t top_window_mismatch(void)
+{
+    for (uint32_t K = 1; K <= 9999u; K++) {
+        const int DK = (K >= 1000u) ? 4 : (K >= 100u) ? 3 : (K >= 10u) ? 2 : 1;
+        uint32_t best = 0;
+        for (uint32_t k = td_pow10(DK - 1); k <= K; k++) {
+            const uint32_t t = td_reverse(k, DK);
+            best = t > best ? t : best;
+        }
+        if (td_top_window(K) != best + 1u) return K;
+    }
+    return 0;
 }

@@ -35,6 +35,7 @@ def model(tmp_path_factory):
     L.gaps_rule_mismatches.restype = C.c_int
     L.first_hit_mismatches.argtypes = [C.c_uint32] * 6
     L.first_hit_mismatches.restype = C.c_int
+    L.top_window_mismatch.restype = C.c_uint32
     return L
 
 
@@ -143,3 +144,10 @@ def test_windowed_first_hit_brute_force(model):
         W = rng.choice([1, 2, 10, 100, 1000, p, rng.randrange(1, p)])
         c0 = rng.randrange(0, p)
         assert model.first_hit_mismatches(p, sig, T, W, c0, 40) == 0, (p, sig, T, W, c0)
+
+
+def test_top_class_window_brute_force(model):
+    """The top rev class searches t = rev(k) only up to the largest reversal
+    of a k <= K (td_top_window: K = 100 leaves only k = 100, t = 1): checked
+    against brute force for every K from 1 to 9999."""
+    assert model.top_window_mismatch() == 0
