@@ -37,9 +37,12 @@ DJ = 4
 # >= 2^16 (the units are counted exactly on the host by trial_div_work, with
 # the lattice kernel's own per-prime routine):
 #   modmul  32-bit Montgomery multiplication: 4 multiply-class + 4 add-class;
+#   modmul_lazy  the same without the final subtraction (the powers of ten
+#           of the setup, primes below 2^30): 4 multiply-class + 2 add-class;
 #   level   one continued-fraction level of a first-hit search: two unsigned
 #           divisions by the same divisor (10 ops each: reciprocal estimate,
-#           multiply-back, correct) and 25 compare/select/add ops;
+#           multiply-back, correct) and 26 compare/select/add ops (the
+#           search-window bound included);
 #   query   start and finish of a first-hit search (residue, bound check,
 #           next start): 10;
 #   hit     k and j decoded from a hit and the bit marked: 12;
@@ -55,8 +58,8 @@ DJ = 4
 #   kstep   one k of the split k loop (primes from 2^16 below the lattice
 #           range, beside the lattice kernel on the second stream): two
 #           residues advanced and compared, 8.
-TD_OPS_PER_UNIT = {"modmul": 8, "level": 45, "query": 10, "hit": 12, "step": 4, "gapstep": 8,
-                   "gapcf": 38, "kstep": 8}
+TD_OPS_PER_UNIT = {"modmul": 8, "modmul_lazy": 6, "level": 46, "query": 10, "hit": 12, "step": 4,
+                   "gapstep": 8, "gapcf": 38, "kstep": 8}
 INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12      # 39.32 Tops/s (256 CUs, wave64, 2.4 GHz)
 # HBM bytes per launch of the lattice kernel on this workload, measured with
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 256 MiB copy kernel's

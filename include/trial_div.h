@@ -142,6 +142,8 @@ enum {
     TD_WORK_KSTEP,       /* prime x k steps of the k-loop kernels (>= 2^16) */
     TD_WORK_GAPSTEP,     /* return-time steps from hit to hit               */
     TD_WORK_GAPCF,       /* division steps computing those return times     */
+    TD_WORK_MODMUL_LAZY, /* Montgomery multiplications without the final    */
+                         /* subtraction (primes below 2^30)                 */
     TD_WORK_FIELDS
 };
 int trial_div_work(const uint32_t *primes, uint32_t n_primes,

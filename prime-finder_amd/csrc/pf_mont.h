@@ -9,10 +9,13 @@
 #define PF_HD inline
 #endif
 
-// Work counter for host-side models (a no-op in the library): one call per
-// Montgomery multiplication.
+// Work counters for host-side models (no-ops in the library): one call per
+// Montgomery multiplication, fully or lazily reduced.
 #ifndef PF_MONT_COUNT
 #define PF_MONT_COUNT() ((void)0)
+#endif
+#ifndef PF_MONT_COUNT_LAZY
+#define PF_MONT_COUNT_LAZY() PF_MONT_COUNT()
 #endif
 
 // High 32 bits of a 32 x 32-bit product (v_mul_hi_u32 on the device).
@@ -75,6 +78,19 @@ PF_HD uint32_t pf_mont_mul(uint32_t a, uint32_t b, const PfMont &m)
     return r >= m.p ? r - m.p : r;
 }
 
+// a * b * R^{-1} mod p without the final subtraction, for p < 2^30 and
+// a, b < 2p: a b < 4 p^2 <= p R, so (a b + q p) / R < 2 p and the result
+// stays a valid operand of the next lazy product.
+PF_HD uint32_t pf_mont_mul_lazy(uint32_t a, uint32_t b, const PfMont &m)
+{
+    PF_MONT_COUNT_LAZY();
+    const uint64_t t = (uint64_t)a * b;
+    const uint32_t lo = (uint32_t)t;
+    const uint32_t hi = (uint32_t)(t >> 32);
+    const uint32_t q = lo * m.nprime;
+    return hi + pf_mulhi(q, m.p) + (lo != 0u ? 1u : 0u);
+}
+
 PF_HD uint32_t pf_mont_in(uint32_t x, const PfMont &m) { return pf_mont_mul(x, m.r2, m); }
 
 PF_HD uint32_t pf_mont_out(uint32_t x, const PfMont &m) { return pf_mont_mul(x, 1u, m); }
@@ -97,7 +113,9 @@ PF_HD uint32_t pf_mont_pow(uint32_t base, uint64_t e, const PfMont &m)
 // lane, sharing the squarings of base: right to left over the bits of the
 // largest exponent, one squaring per bit and one product per set bit after
 // an exponent's lowest (which takes the power itself: R * sq / R = sq).
-template <int NE>
+// LAZY (only for p < 2^30): the products skip their final subtraction
+// (pf_mont_mul_lazy) and each power is reduced once at the end.
+template <int NE, bool LAZY = false>
 PF_HD void pf_mont_pow_multi(uint32_t base, const uint64_t (&e)[NE], uint32_t (&out)[NE],
                              const PfMont &m)
 {
@@ -110,7 +128,13 @@ PF_HD void pf_mont_pow_multi(uint32_t base, const uint64_t (&e)[NE], uint32_t (&
     for (int i = 0; (top >> i) != 0; i++) {
         for (int k = 0; k < NE; k++)
             if ((e[k] >> i) & 1u)
-                out[k] = (e[k] & ((1ull << i) - 1u)) == 0u ? sq : pf_mont_mul(out[k], sq, m);
-        if ((top >> (i + 1)) != 0) sq = pf_mont_mul(sq, sq, m);
+                out[k] = (e[k] & ((1ull << i) - 1u)) == 0u ? sq
+                         : LAZY ? pf_mont_mul_lazy(out[k], sq, m) : pf_mont_mul(out[k], sq, m);
+        if ((top >> (i + 1)) != 0) sq = LAZY ? pf_mont_mul_lazy(sq, sq, m) : pf_mont_mul(sq, sq, m);
     }
+    if (LAZY)
+        for (int k = 0; k < NE; k++) {
+            const uint32_t d = out[k] - m.p;
+            out[k] = d < out[k] ? d : out[k];
+        }
 }

@@ -455,12 +455,23 @@ PF_HD void td_lattice_setup(uint32_t p, long long a, long long b, long long c, u
     // one exponent |b - c| in each chain.
     const long long eb1 = a - b - 1;
     const uint64_t bc = (uint64_t)(c <= b ? b - c : c - b);
+    // (lazily reduced products where every prime of the wave is below 2^30)
     uint32_t pw[2], nw[3];
     {
         const uint64_t pe[2] = {(uint64_t)a, bc};
         const uint64_t ne[3] = {(uint64_t)c, (uint64_t)(a - c - DJ), bc};
-        pf_mont_pow_multi<2>(ten, pe, pw, m);
-        pf_mont_pow_multi<3>(tenInv, ne, nw, m);
+#if defined(__HIP_DEVICE_COMPILE__)
+        const bool lazy = __ballot(p >= (1u << 30)) == 0ull;
+#else
+        const bool lazy = p < (1u << 30);
+#endif
+        if (lazy) {
+            pf_mont_pow_multi<2, true>(ten, pe, pw, m);
+            pf_mont_pow_multi<3, true>(tenInv, ne, nw, m);
+        } else {
+            pf_mont_pow_multi<2>(ten, pe, pw, m);
+            pf_mont_pow_multi<3>(tenInv, ne, nw, m);
+        }
     }
     const uint32_t raM = pw[0];                                                  // 10^a
     const uint32_t icM = nw[0];                                                  // 10^-c

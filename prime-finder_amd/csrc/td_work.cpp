@@ -15,12 +15,13 @@
 
 namespace {
 struct TdWork {
-    uint64_t prime, modmul, query, level, hit, step, gapstep, gapcf;
+    uint64_t prime, modmul, query, level, hit, step, gapstep, gapcf, modmul_lazy;
 };
 thread_local TdWork *td_work_cur;
 }  // namespace
 
 #define PF_MONT_COUNT() (++td_work_cur->modmul)
+#define PF_MONT_COUNT_LAZY() (++td_work_cur->modmul_lazy)
 #define TD_LAT_COUNT(what) (++td_work_cur->what)
 #include "td_lattice.h"
 #include "../../include/trial_div.h"
@@ -64,7 +65,7 @@ extern "C" int trial_div_work(const uint32_t *primes, uint32_t n_primes, long lo
     out[TD_WORK_KSTEP] = ksteps;
     if (!merged) return 0;
     const unsigned nt = threads ? threads : 1u;
-    std::vector<TdWork> part(nt, TdWork{0, 0, 0, 0, 0, 0, 0, 0});
+    std::vector<TdWork> part(nt, TdWork{0, 0, 0, 0, 0, 0, 0, 0, 0});
     std::vector<std::thread> pool;
     for (unsigned t = 0; t < nt; t++)
         pool.emplace_back([&, t] {
@@ -85,6 +86,7 @@ extern "C" int trial_div_work(const uint32_t *primes, uint32_t n_primes, long lo
         out[TD_WORK_STEP] += w.step;
         out[TD_WORK_GAPSTEP] += w.gapstep;
         out[TD_WORK_GAPCF] += w.gapcf;
+        out[TD_WORK_MODMUL_LAZY] += w.modmul_lazy;
     }
     return 0;
 }

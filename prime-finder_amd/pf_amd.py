@@ -142,13 +142,14 @@ def trial_div_gpu_kj(primes: np.ndarray, a: int, b: int, c: int, K: int, J: int,
 
 
 TD_WORK_FIELDS = ("prime", "modmul", "query", "level", "hit", "step", "kstep", "gapstep",
-                  "gapcf")
+                  "gapcf", "modmul_lazy")
 
 
 def trial_div_work(primes: np.ndarray, a: int, b: int, c: int, K: int, J: int,
                    threads: int = 1) -> dict:
     """Algorithmic work of one launch on (a, b, c) over `primes`, counted on
-    the host (trial_div_work): lattice primes, Montgomery multiplications,
+    the host (trial_div_work): lattice primes, Montgomery multiplications
+    (fully, and in "modmul_lazy" lazily, reduced),
     first-hit queries, continued-fraction levels, hits, one-digit-class
     steps, k-loop prime x k steps and return-time steps between hits."""
     p = _u32(primes)
