@@ -180,17 +180,20 @@ PF_HD uint32_t td_window_thr(uint32_t p, uint32_t W)
 PF_HD void td_fh_run(TdFh &q, uint32_t T, uint32_t athr)
 {
     if (!q.live) return;
+    // (the sign and the outcome are kept as 32-bit values, not booleans, so
+    // that they stay in vector registers: lane-mask booleans carried across
+    // the divergent exits cost several scalar mask operations per level)
     uint32_t L0 = q.L, A0 = q.A, M = q.M, tacc = q.tacc, w = 0u;
-    bool neg = q.neg, live = true, fail = false;
+    uint32_t negm = q.neg ? ~0u : 0u;    // all ones: this level's value counts negated
+    uint32_t st = 0u;                    // 1: decided, 2: no hit, 0: out of levels
     for (int it = 0; it < TD_FH_MAX_LEVELS; it++) {
         TD_LAT_COUNT(level);
         const uint32_t MA = M - A0;
-        const bool mir = A0 > MA;        // mirror: (M - A) x mod M in [M - R, M - L]
-        const uint32_t A = mir ? MA : A0;
-        const uint32_t L = mir ? M - (L0 + (T - 1u)) : L0;
-        const uint32_t sM = neg ? 0u - M : M;
-        tacc += mir ? sM : 0u;
-        neg = neg != mir;
+        const uint32_t mirm = A0 > MA ? ~0u : 0u;   // mirror: (M - A) x mod M in [M - R, M - L]
+        const uint32_t A = mirm ? MA : A0;
+        const uint32_t L = mirm ? M - (L0 + (T - 1u)) : L0;
+        tacc += ((M ^ negm) - negm) & mirm;          // +-M when mirrored
+        negm ^= mirm;
         const uint32_t Ad = A ? A : 1u;
         const float rA = td_rcp_lo(Ad);
         uint32_t eL, eM;
@@ -209,28 +212,24 @@ PF_HD void td_fh_run(TdFh &q, uint32_t T, uint32_t athr)
         }
         const uint32_t up = rL ? Ad - rL : 0u;   // distance to the next multiple of A
         if (up < T || A == 0u) {                 // decided: a multiple lands in [L, R]
-            TD_LAT_COUNT(query);
-            live = false;
-            fail = A == 0u;
+            st = A ? 1u : 2u;
             w = L + up;
             break;
         }
-        tacc += neg ? rL - L : L - rL;
+        tacc += ((L - rL) ^ negm) - negm;        // +-(L - L mod A)
         M = A;
         A0 = A - mA;
         L0 = rL;
         if (M <= athr) {                         // first hit >= p / A >= W
-            TD_LAT_COUNT(query);
-            live = false;
-            fail = true;
+            st = 2u;
             break;
         }
     }
-    if (live) TD_LAT_COUNT(query);
+    TD_LAT_COUNT(query);
     q.tacc = tacc;
-    q.neg = neg;
+    q.neg = negm != 0u;
     q.w = w;
-    q.fail = fail || live;
+    q.fail = st != 1u;
     q.live = false;
 }
 
@@ -299,22 +298,23 @@ PF_HD uint32_t td_div_est(uint32_t x, uint32_t Y, float rY, bool &slow)
 // queries.  False when a run cannot proceed (a composite list entry).
 PF_HD bool td_gaps(uint32_t sig, uint32_t T, uint32_t p, TdGaps &g)
 {
+    // (both sides' updates are selects and the flags 32-bit values, so the
+    // loop body has no divergent branch but the rare exact division)
     uint32_t a = 1u, al = sig, b = 1u, be = p - sig;
-    bool fa = al < T, fb = be < T;
-    g.a = 1u;
-    g.alpha = al;
-    g.b = 1u;
-    g.beta = be;
-    for (int it = 0; it < 64 && !(fa && fb); it++) {
+    uint32_t fa = al < T ? 1u : 0u, fb = be < T ? 1u : 0u, bad = 0u;
+    uint32_t ga = 1u, galpha = al, gb = 1u, gbeta = be;
+    for (int it = 0; it < 64; it++) {
+        if ((fa & fb) | bad) break;
         TD_LAT_COUNT(gapcf);
         const bool up = al > be;                 // reduce alpha's side
-        const uint32_t X = up ? al : be, Y = up ? be : al;
+        const uint32_t X = up ? al : be, Y0 = up ? be : al;
         const uint32_t cX = up ? a : b, cY = up ? b : a;
-        if (Y == 0u) return false;
+        bad = Y0 == 0u ? 1u : 0u;                // (a composite entry)
+        const uint32_t Y = Y0 ? Y0 : 1u;
         const float rY = td_rcp_lo(Y);
         bool slow = false;
         uint32_t k = td_div_est(X - 1u, Y, rY, slow);          // length of the run
-        const bool need = up ? !fa : !fb;                       // X >= T
+        const bool need = (up ? fa : fb) == 0u;                 // X >= T
         uint32_t j = td_div_est(need ? X - T : 0u, Y, rY, slow) + 1u;
 #if defined(__HIP_DEVICE_COMPILE__)
         if (__ballot(slow) != 0ull) {
@@ -326,26 +326,27 @@ PF_HD bool td_gaps(uint32_t sig, uint32_t T, uint32_t p, TdGaps &g)
                 j = (need ? X - T : 0u) / Y + 1u;
             }
         }
-        if (need && j <= k) {                    // this side's first return, inside the run
-            if (up) {
-                fa = true;
-                g.a = cX + j * cY;
-                g.alpha = X - j * Y;
-            } else {
-                fb = true;
-                g.b = cX + j * cY;
-                g.beta = X - j * Y;
-            }
-        }
-        if (up) {
-            al = X - k * Y;
-            a = cX + k * cY;
-        } else {
-            be = X - k * Y;
-            b = cX + k * cY;
-        }
+        // this side's first return, when it lies inside the run
+        const bool found = need && j <= k && !bad;
+        const uint32_t nJ = cX + j * cY, dJ = X - j * Y;
+        const bool fA = found && up, fB = found && !up;
+        ga = fA ? nJ : ga;
+        galpha = fA ? dJ : galpha;
+        gb = fB ? nJ : gb;
+        gbeta = fB ? dJ : gbeta;
+        fa |= fA ? 1u : 0u;
+        fb |= fB ? 1u : 0u;
+        const uint32_t nK = cX + k * cY, dK = X - k * Y;
+        al = up ? dK : al;
+        a = up ? nK : a;
+        be = up ? be : dK;
+        b = up ? b : nK;
     }
-    return fa && fb;
+    g.a = ga;
+    g.alpha = galpha;
+    g.b = gb;
+    g.beta = gbeta;
+    return (fa & fb) != 0u && !bad;
 }
 
 // ---- per-prime setup and conditions -----------------------------------
@@ -574,18 +575,13 @@ PF_HD void td_lattice_run(const TdLatRec &r, int cond, const TdLatItem &it, uint
     const uint32_t da = g.a < it.W ? g.a : it.W;
     const uint32_t db = g.b < it.W ? g.b : it.W;
     const uint32_t dab = (uint64_t)g.a + g.b < it.W ? g.a + g.b : it.W;
+    const uint32_t ya = it.T - g.alpha;
     for (;;) {
         TD_LAT_COUNT(gapstep);
-        if (y < it.T - g.alpha) {
-            t += da;
-            y += g.alpha;
-        } else if (y >= g.beta) {
-            t += db;
-            y -= g.beta;
-        } else {
-            t += dab;
-            y = y + g.alpha - g.beta;
-        }
+        // (selects, not branches: y + alpha, y - beta or y + alpha - beta)
+        const bool s1 = y < ya, s2 = y >= g.beta;
+        t += s1 ? da : s2 ? db : dab;
+        y += s1 ? g.alpha : s2 ? 0u - g.beta : g.alpha - g.beta;
         if (t >= it.W) return;
         td_lattice_mark<DJ>(cond, t, y, K, J, mark);
     }
