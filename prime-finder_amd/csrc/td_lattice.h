@@ -199,16 +199,9 @@ PF_HD void td_fh_run(TdFh &q, uint32_t T, uint32_t athr)
         uint32_t eL, eM;
         uint32_t rL = td_mod_est(L, Ad, rA, eL);
         uint32_t mA = td_mod_est(M, Ad, rA, eM);
-        const bool slow = (eL | eM) >= (1u << 18);
-#if defined(__HIP_DEVICE_COMPILE__)
-        if (__ballot(slow) != 0ull) {
-#else
-        {
-#endif
-            if (slow) {
-                rL = L % Ad;
-                mA = M % Ad;
-            }
+        if ((eL | eM) >= (1u << 18)) {
+            rL = L % Ad;
+            mA = M % Ad;
         }
         const uint32_t up = rL ? Ad - rL : 0u;   // distance to the next multiple of A
         if (up < T || A == 0u) {                 // decided: a multiple lands in [L, R]
@@ -316,15 +309,9 @@ PF_HD bool td_gaps(uint32_t sig, uint32_t T, uint32_t p, TdGaps &g)
         uint32_t k = td_div_est(X - 1u, Y, rY, slow);          // length of the run
         const bool need = (up ? fa : fb) == 0u;                 // X >= T
         uint32_t j = td_div_est(need ? X - T : 0u, Y, rY, slow) + 1u;
-#if defined(__HIP_DEVICE_COMPILE__)
-        if (__ballot(slow) != 0ull) {
-#else
-        {
-#endif
-            if (slow) {
-                k = (X - 1u) / Y;
-                j = (need ? X - T : 0u) / Y + 1u;
-            }
+        if (slow) {
+            k = (X - 1u) / Y;
+            j = (need ? X - T : 0u) / Y + 1u;
         }
         // this side's first return, when it lies inside the run
         const bool found = need && j <= k && !bad;
