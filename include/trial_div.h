@@ -97,7 +97,7 @@ int trial_div_launch_kj_ev(const uint32_t *d_primes, uint32_t n_primes,
 /* Synchronises the current device, then returns and clears the error latched
  * by earlier launches: 0 = none, else a bit mask: 1 = a prime outside
  * [2, 2^31), 2 = more than 65536 list entries below 2^16, 4 = more than
- * 1 100 000 entries from 2^16 to the lattice range (2^18) (both: a
+ * 1 100 000 entries from 2^16 to the lattice range (2^17) (both: a
  * duplicated list). */
 int trial_div_check_error(void);
 
@@ -112,7 +112,7 @@ int trial_div_check_error(void);
  * each GPU the library runs on holds until the process ends; every kernel
  * runs in order on `stream`.  Limits: K, J in 1..9999 (-1 and a message
  * otherwise), at most 65536 list entries below 2^16 and 1 100 000 from 2^16
- * to 2^18 (latched, reported by trial_div_check_error() as 2 and 4).  All
+ * to 2^17 (latched, reported by trial_div_check_error() as 2 and 4).  All
  * calls on one GPU share the array:
  * calls that overlap in time on one GPU (other streams, other host threads)
  * race and must be serialised by the caller.  Same arguments, results and

@@ -1021,7 +1021,7 @@ void td_mid_kernel(const uint32_t *__restrict__ mid, const uint32_t *__restrict_
 // lanes but cost more in bookkeeping than it saved: 2.29 against 1.92 ms
 // per 4096 x 4096 triple over the primes below 10^8.)
 #ifndef TD_LAT_MIN_LOG2
-#define TD_LAT_MIN_LOG2 18
+#define TD_LAT_MIN_LOG2 17
 #endif
 constexpr uint32_t TD_LAT_MIN = 1u << TD_LAT_MIN_LOG2;
 static_assert(TD_LAT_MIN >= TD_SMALL_LIMIT, "the lattice path needs p > 10^4 >= J");
