@@ -63,9 +63,9 @@ TD_OPS_PER_UNIT = {"modmul": 8, "modmul_lazy": 6, "level": 46, "query": 10, "hit
 INT32_VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12      # 39.32 Tops/s (256 CUs, wave64, 2.4 GHz)
 # HBM bytes per launch of the lattice kernel on this workload, measured with
 # rocprofv3 FETCH_SIZE and WRITE_SIZE (scaled by a 256 MiB copy kernel's
-# readings: x2.0 and x1.0): 23.30 MB read (the prime list) + 417.82 MB
-# written (memory-side atomic ORs of the hits).
-TD_TRAFFIC_BYTES = 23.30e6 + 417.82e6
+# readings: x2.0 and x1.0): 23.29 MB read (the prime list) + 478.65 MB
+# written (memory-side atomic ORs of the hits of the primes from 2^17).
+TD_TRAFFIC_BYTES = 23.29e6 + 478.65e6
 # Same for one batched squaring step of the 50 000-digit Miller-Rabin batch
 # (256 bigints, 2^14 transforms): 91.7 MB read + 14.4 MB written.
 MR_TRAFFIC_BYTES_PER_STEP = 91.7e6 + 14.4e6
