@@ -291,8 +291,8 @@ PF_HD uint32_t td_div_est(uint32_t x, uint32_t Y, float rY, bool &slow)
 // queries.  False when a run cannot proceed (a composite list entry).
 PF_HD bool td_gaps(uint32_t sig, uint32_t T, uint32_t p, TdGaps &g)
 {
-    // (both sides' updates are selects and the flags 32-bit values, so the
-    // loop body has no divergent branch but the rare exact division)
+    // (the flags are 32-bit values and the runs' updates selects; only the
+    // rare first return and exact division take a branch)
     uint32_t a = 1u, al = sig, b = 1u, be = p - sig;
     uint32_t fa = al < T ? 1u : 0u, fb = be < T ? 1u : 0u, bad = 0u;
     uint32_t ga = 1u, galpha = al, gb = 1u, gbeta = be;
@@ -307,23 +307,27 @@ PF_HD bool td_gaps(uint32_t sig, uint32_t T, uint32_t p, TdGaps &g)
         const float rY = td_rcp_lo(Y);
         bool slow = false;
         uint32_t k = td_div_est(X - 1u, Y, rY, slow);          // length of the run
-        const bool need = (up ? fa : fb) == 0u;                 // X >= T
-        uint32_t j = td_div_est(need ? X - T : 0u, Y, rY, slow) + 1u;
-        if (slow) {
-            k = (X - 1u) / Y;
-            j = (need ? X - T : 0u) / Y + 1u;
+        if (slow) k = (X - 1u) / Y;
+        const uint32_t nK = cX + k * cY, dK = X - k * Y;        // the run's last value
+        // this side's first return lies inside the run iff the run ends
+        // below T (X >= T while the side is open); its position j is only
+        // computed then (rare: a branch the wave mostly skips).  j's
+        // estimate is at most k's, so it is exact when k's was.
+        const bool need = (up ? fa : fb) == 0u;
+        if (need && dK < T && !bad) {
+            bool s2 = slow;
+            const uint32_t j = s2 ? (X - T) / Y + 1u : td_div_est(X - T, Y, rY, s2) + 1u;
+            const uint32_t nJ = cX + j * cY, dJ = X - j * Y;
+            if (up) {
+                fa = 1u;
+                ga = nJ;
+                galpha = dJ;
+            } else {
+                fb = 1u;
+                gb = nJ;
+                gbeta = dJ;
+            }
         }
-        // this side's first return, when it lies inside the run
-        const bool found = need && j <= k && !bad;
-        const uint32_t nJ = cX + j * cY, dJ = X - j * Y;
-        const bool fA = found && up, fB = found && !up;
-        ga = fA ? nJ : ga;
-        galpha = fA ? dJ : galpha;
-        gb = fB ? nJ : gb;
-        gbeta = fB ? dJ : gbeta;
-        fa |= fA ? 1u : 0u;
-        fb |= fB ? 1u : 0u;
-        const uint32_t nK = cX + k * cY, dK = X - k * Y;
         al = up ? dK : al;
         a = up ? nK : a;
         be = up ? be : dK;
