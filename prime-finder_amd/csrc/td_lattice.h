@@ -311,12 +311,13 @@ PF_HD bool td_gaps(uint32_t sig, uint32_t T, uint32_t p, TdGaps &g)
         const uint32_t nK = cX + k * cY, dK = X - k * Y;        // the run's last value
         // this side's first return lies inside the run iff the run ends
         // below T (X >= T while the side is open); its position j is only
-        // computed then (rare: a branch the wave mostly skips).  j's
-        // estimate is at most k's, so it is exact when k's was.
+        // computed then (rare: a branch the wave mostly skips), exactly
+        // when either estimate reached 2^18.
         const bool need = (up ? fa : fb) == 0u;
         if (need && dK < T && !bad) {
             bool s2 = slow;
-            const uint32_t j = s2 ? (X - T) / Y + 1u : td_div_est(X - T, Y, rY, s2) + 1u;
+            uint32_t j = td_div_est(X - T, Y, rY, s2) + 1u;
+            if (s2) j = (X - T) / Y + 1u;
             const uint32_t nJ = cX + j * cY, dJ = X - j * Y;
             if (up) {
                 fa = 1u;
