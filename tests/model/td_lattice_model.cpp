@@ -119,3 +119,39 @@ extern "C" uint32_t top_window_mismatch(void)
     }
     return 0;
 }
+
+// The lazily reduced shared-squaring powers (pf_mont_pow_multi<NE, true>,
+// products without the final subtraction, p < 2^30) against the fully
+// reduced ones and against plain square-and-multiply in 64-bit arithmetic,
+// for n random exponent triples below 2^20 (seeded xorshift).  Returns the
+// number of mismatches.
+extern "C" int lazy_pow_mismatches(uint32_t p, uint32_t seed, uint32_t n)
+{
+    const PfMont m = pf_mont_init(p);
+    uint64_t st = 0x9E3779B97F4A7C15ull ^ seed;
+    auto next = [&]() {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return st;
+    };
+    int bad = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t base = (uint32_t)(next() % (p - 1u)) + 1u;
+        const uint64_t e[3] = {next() % (1u << 20), next() % (1u << 20), i % 7u};
+        uint32_t lazy[3], full[3];
+        const uint32_t bM = pf_mont_in(base, m);
+        pf_mont_pow_multi<3, true>(bM, e, lazy, m);
+        pf_mont_pow_multi<3>(bM, e, full, m);
+        for (int k = 0; k < 3; k++) {
+            uint64_t r = 1, b = base, x = e[k];
+            while (x) {
+                if (x & 1u) r = r * b % p;
+                b = b * b % p;
+                x >>= 1;
+            }
+            if (lazy[k] != full[k] || pf_mont_out(lazy[k], m) != (uint32_t)r) bad++;
+        }
+    }
+    return bad;
+}

@@ -36,6 +36,8 @@ def model(tmp_path_factory):
     L.first_hit_mismatches.argtypes = [C.c_uint32] * 6
     L.first_hit_mismatches.restype = C.c_int
     L.top_window_mismatch.restype = C.c_uint32
+    L.lazy_pow_mismatches.argtypes = [C.c_uint32] * 3
+    L.lazy_pow_mismatches.restype = C.c_int
     return L
 
 
@@ -151,3 +153,13 @@ def test_top_class_window_brute_force(model):
     of a k <= K (td_top_window: K = 100 leaves only k = 100, t = 1): checked
     against brute force for every K from 1 to 9999."""
     assert model.top_window_mismatch() == 0
+
+
+def test_lazy_powers_equal_reduced_powers(model):
+    """The setup's powers with lazily reduced products (no final
+    subtraction, operands below 2p, for p < 2^30) equal the fully reduced
+    chain and plain 64-bit square-and-multiply, on odd moduli up to just
+    below 2^30 (primes and composites: the bound needs only p odd and
+    4 p^2 <= p 2^32) and exponents up to 2^20, zero included."""
+    for p in (3, 10007, 65537, 999999937, (1 << 30) - 35, (1 << 30) - 1, 1000001):
+        assert model.lazy_pow_mismatches(p, p & 0xFFFF, 400) == 0, p
